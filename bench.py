@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark: aggregate MD steps/s of the batched 22-replica SDM lambda ladder at 22,750 atoms.
+
+Workload (one "step" = one SDM Langevin step of every replica, inputs resident in HBM):
+  22 replicas of the 22,750-atom host-guest system (7,500 water-like triatomics, 200-atom
+  host shell, 50-atom guest displaced by (2.5,0,0) nm), 6.1 nm periodic box, LJ + reaction
+  field (rc 1.0 nm, eps_rf 78.5), ILogistic bias (alpha 0.1, u0 110), rational soft core,
+  lambda ladder 0->0.5 (11 replicas) and 0.5->0 (11 replicas), 1 fs, 300 K, 1/ps friction.
+  Synthetic seeded coordinates (no datasets); every replica has its own coordinates.
+
+Printed: one JSON line (rank 0).  `value` = aggregate steps/s of all replicas (22 x steps /
+elapsed).  Also reported: the single-replica plain path (R=1) steps/s and ns/day, the
+batched/plain ratio, the pair kernel's roofline (FP32 VALU; 50 FLOP per in-cutoff pair)
+timed live with HIP events on the library stream, and the FP64 CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 50            # FMA = 2; see the roofline accounting in the design notes
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 vector (published)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--replicas", type=int, default=22)
+    ap.add_argument("--single-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "k_pair_config3_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(n_steps):
+    """FP64 Reference-form step (3 evaluations per step, single thread) on one replica."""
+    import numpy as np
+    from openmm_sdm_plugin_amd import systems
+    from oracle import oracle as O
+    s = systems.make_config2(frame=0)
+    p = systems.params_config2()
+    topo = O.Topology(s)
+    op = O.make_params(p)
+    x, v = s.positions.copy(), s.velocities.copy()
+    x, v, t, st, _ = O.ref_step(topo, op, x, v, 0.0, 0, p.seed, 0, 1)  # warm-up
+    t0 = time.perf_counter()
+    x, v, t, st, _ = O.ref_step(topo, op, x, v, t, st, p.seed, 0, n_steps)
+    dt = time.perf_counter() - t0
+    assert np.all(np.isfinite(x))
+    return {"value": n_steps / dt, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n_steps} steps (after 1 warm-up) of the 22,750-atom replica with the "
+                      "FP64 oracle's Reference-platform step (E at x, E at x+d, bonded, "
+                      "execute, Langevin), single thread; 22 sequential replicas give the "
+                      "same aggregate steps/s",
+            "seconds": dt}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the replica ladder would shard by replica; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from openmm_sdm_plugin_amd import _native, systems
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    R = a.replicas
+    frames = [systems.make_config2(frame=k) for k in range(R)]
+    params = systems.params_config3(R)
+    ctx = _native.NativeContext(frames[0], R)
+    for r in range(R):
+        ctx.set_params(r, params[r])
+        ctx.set_positions(r, frames[r].positions)
+        ctx.set_velocities(r, frames[r].velocities)
+    ctx.set_u_history(100, 4096)
+    tuning = ctx.autotune(20) if not a.no_autotune else ctx.get_tuning()
+
+    # ---- batched ladder: warm-up then exactly K timed steps ----
+    ctx.step(a.warmup)
+    ctx.energies()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    e = ctx.energies()  # synchronises the library stream
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    agg = R * a.steps / elapsed
+    flags = [x["flags"] for x in e]
+    ok = all(np.isfinite(x["pot_energy"]) for x in e) and not any(flags)
+
+    # ---- roofline of the pair kernel (live HIP events on the library stream) ----
+    p0_a, _ = ctx.count_pairs()
+    pair_ms = ctx.time_pair_kernel(20)
+    p0_b, _ = ctx.count_pairs()
+    pairs = 0.5 * (float(p0_a.sum()) + float(p0_b.sum()))
+    achieved = FLOP_PER_PAIR * pairs / (pair_ms * 1e-3) / 1e12
+    traffic = read_traffic()
+
+    # ---- plain single-replica path (R = 1) ----
+    one = _native.NativeContext(frames[0], 1)
+    one.set_params(0, params[R // 2] if R > 1 else params[0])
+    one.set_positions(0, frames[0].positions)
+    one.set_velocities(0, frames[0].velocities)
+    if not a.no_autotune:
+        one.autotune(20)
+    one.step(a.warmup)
+    one.energies()
+    t1 = time.perf_counter()
+    one.step(a.single_steps)
+    one.energies()
+    single = a.single_steps / (time.perf_counter() - t1)
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a.cpu_steps)
+
+    out = {
+        "metric": "aggregate MD steps/s across batched lambda-replicas at 23k atoms; "
+                  "single-replica ns/day",
+        "value": agg,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded coordinates, random-draw guest/host parameters)",
+        "config": {
+            "workload": "config 3: 22-replica lambda ladder of the 22,750-atom host-guest "
+                        "system, 6.1 nm box, LJ+RF rc 1.0 nm, 1 fs Langevin",
+            "replicas": R, "atoms": frames[0].n, "arithmetic": "pair math f32, int64 "
+            "fixed-point forces, f64 energy accumulation, f64 displaced-state terms and "
+            "integration", "tuning": tuning,
+        },
+        "single_replica_steps_per_s": single,
+        "single_replica_ns_per_day": single * 0.001 * 86400.0 / 1000.0,
+        "batched_over_plain": agg / single,
+        "all_finite_no_flags": ok,
+        "roofline": {
+            "bound": "valu", "kernel": "k_pair", "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": traffic, "pair_kernel_ms": pair_ms,
+            "pairs_per_launch": pairs, "flop_per_pair": FLOP_PER_PAIR,
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = agg / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * sdm_hip.h -- C ABI of the MI355X (gfx950) SDM alchemical force + Langevin step.
+ *
+ * The reference plugin (rajatkrpal/openmm_sdm_plugin) exposes one OpenMM platform
+ * kernel, IntegrateLangevinStepSDMKernel ("IntegrateLangevinStepSDM",
+ * openmmapi/include/SDMKernels.h:47-113), created by a platform kernel factory that
+ * OpenMM reaches through the plugin's exported registerKernelFactories()
+ * (platforms/reference/src/ReferenceSDMKernelFactory.cpp:41-63).  Its step() drives
+ * two NonbondedForce evaluations (x and x+d), one bonded evaluation and execute()
+ * (openmmapi/src/LangevinIntegratorSDM.cpp:153-183).  This header is the boundary that
+ * replaces that kernel: plain pointers and sizes, no C++ or torch types, int status
+ * codes (0 = OK) with sdm_last_error() for the message.  Every call is made on the
+ * calling thread's current HIP device; a context owns its device buffers and one HIP
+ * stream and is not thread-safe.  Units: nm, ps, kJ/mol, e, amu, K.
+ *
+ * One context holds R >= 1 replicas of the same topology (a lambda ladder); each
+ * replica has its own coordinates, velocities, parameters and clock.  R == 1 is the
+ * plain single-replica kernel.
+ */
+#ifndef SDM_HIP_H
+#define SDM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define SDM_OK 0
+#define SDM_ERR_INVALID -1  /* bad argument (message in sdm_last_error) */
+#define SDM_ERR_HIP -2      /* HIP runtime failure */
+#define SDM_ERR_STATE -3    /* call not valid in the current state */
+#define SDM_ERR_NUMERIC -4  /* non-finite energy or fixed-point overflow flagged */
+#define SDM_ERR_CAPACITY -5 /* neighbour-structure capacity exceeded */
+#define SDM_ERR_UNSUPPORTED -6
+
+/* ---- method constants: LangevinIntegratorSDM.h:120-122 and :143-145 ---- */
+#define SDM_LINEAR_METHOD 0
+#define SDM_QUADRATIC_METHOD 1
+#define SDM_ILOGISTIC_METHOD 2
+#define SDM_NO_SOFTCORE_METHOD 0
+#define SDM_TANH_METHOD 1
+#define SDM_RATIONAL_METHOD 2
+
+/* ---- per-replica flags reported in sdm_energies.flags ---- */
+#define SDM_FLAG_NONFINITE 1  /* an energy or u' was not finite */
+#define SDM_FLAG_OVERFLOW 2   /* a force exceeded the 2^31 kJ/mol/nm fixed-point range */
+#define SDM_FLAG_CAPACITY 4   /* a neighbour list was truncated */
+
+/* System description: replaces the OpenMM System + NonbondedForce (group 2) +
+   HarmonicBond/AngleForce (group 1) + the integrator's displacement map
+   (LangevinIntegratorSDM.h:467-472, captured at ReferenceSDMKernels.cpp:150-154). */
+typedef struct {
+    int32_t n_atoms;
+    const double* charge;  /* [n_atoms] e */
+    const double* sigma;   /* [n_atoms] nm, Lorentz-Berthelot */
+    const double* epsilon; /* [n_atoms] kJ/mol */
+    const double* mass;    /* [n_atoms] amu; 0 = fixed atom */
+    int32_t n_exclusions;
+    const int32_t* exclusions; /* [2*n_exclusions] excluded nonbonded pairs */
+    int32_t n_bonds;
+    const int32_t* bond_atoms; /* [2*n_bonds] E = k/2 (r-r0)^2 */
+    const double* bond_r0;
+    const double* bond_k;
+    int32_t n_angles;
+    const int32_t* angle_atoms; /* [3*n_angles] E = k/2 (theta-theta0)^2 */
+    const double* angle_theta0;
+    const double* angle_k;
+    const double* displacement; /* [3*n_atoms] state-2 displacement d_i */
+    double box[3];              /* rectangular periodic box */
+    int32_t periodic;           /* must be 1 (CutoffPeriodic) */
+    double cutoff;              /* nm; box >= 2*cutoff in every dimension */
+    double eps_rf;              /* reaction-field dielectric */
+} sdm_system_desc;
+
+/* Integrator parameters of one replica: the LangevinIntegratorSDM fields
+   (LangevinIntegratorSDM.h:497-521, defaults LangevinIntegratorSDM.cpp:48-85). */
+typedef struct {
+    int32_t bias_method;     /* setBiasMethod */
+    int32_t softcore_method; /* setSoftCoreMethod */
+    double lambda;           /* setLambda */
+    double gamma;            /* setGamma */
+    double wbcoeff;          /* setWBcoeff */
+    double w0coeff;          /* setW0coeff */
+    double lambda1, lambda2; /* setLambda1/2 */
+    double alpha;            /* setAlpha, 1/(kJ/mol) */
+    double u0;               /* setU0 */
+    double umax, acore, ubcore;
+    int32_t nonequilibrium;  /* setNonEquilibrium */
+    double noneq_tmax;       /* setNoneqtmax */
+    double work;             /* setNoneqWorkvalue */
+    double m_l1, m_l2, m_u0, m_w0; /* set*Slope */
+    double b_l1, b_l2, b_u0, b_w0; /* set*intercept */
+    double temperature;      /* K */
+    double friction;         /* 1/ps */
+    double step_size;        /* ps */
+    uint64_t seed;           /* setRandomNumberSeed */
+} sdm_params;
+
+/* Observables of one replica for the last evaluated step
+   (ReferenceSDMKernels.cpp:264-302). */
+typedef struct {
+    double U0;         /* State1Energy: group-2 energy at x */
+    double U1;         /* State2Energy: group-2 energy at x+d */
+    double u;          /* U1 - U0, summed over pairs with d_i != d_j */
+    double Ub;         /* BondAndRestraintEnergy: group-1 energy at x */
+    double bind_e;     /* getBindE(): u' = SoftCoreF(u) */
+    double fp;         /* du'/du */
+    double W;          /* bias W(u') */
+    double Wp;         /* W'(u') */
+    double sp;         /* W' * fp: weight of the state-2 force */
+    double pot_energy; /* getPotEnergy(): U0 + W + Ub */
+    double lambda, lambda1, lambda2, u0, w0; /* parameters in effect (noneq schedule) */
+    double work;       /* getNoneqWorkvalue() after the step */
+    double time;       /* clock after the step (ps) */
+    int64_t step;      /* steps taken */
+    int32_t flags;     /* SDM_FLAG_* (sticky until sdm_clear_flags) */
+    int32_t pad;
+} sdm_energies;
+
+typedef struct sdm_context sdm_context;
+
+/* Launch configuration of the hot kernels, chosen by sdm_autotune. */
+typedef struct {
+    int32_t pair_waves_per_group; /* 1, 2 or 4 wave64s per workgroup in the pair kernel */
+    int32_t cell_cells_per_dim;   /* sort-grid resolution; 0 = automatic */
+    int32_t use_graph;            /* replay steps from a captured hipGraph */
+    int32_t reserved;
+} sdm_tuning;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sdm_last_error(void);
+
+/* Creates a context for n_replicas replicas of `sys` on the current device.
+   Replaces IntegrateLangevinStepSDMKernel::initialize (SDMKernels.h:60;
+   ReferenceSDMKernels.cpp:144-159, OpenCLSDMKernels.cpp:73-133): masses and the
+   displacement map are captured here.  All replicas start with the reference
+   constructor defaults (LangevinIntegratorSDM.cpp:48-85), zero coordinates and
+   velocities. */
+int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out);
+int sdm_destroy(sdm_context* ctx);
+int sdm_num_replicas(const sdm_context* ctx, int32_t* n_replicas, int32_t* n_atoms);
+
+/* Parameters: LangevinIntegratorSDM setters/getters (LangevinIntegratorSDM.h:81-464).
+   replica = -1 sets every replica.  sdm_get_params returns the device copy, which the
+   nonequilibrium schedule and work accumulation update every step. */
+int sdm_set_params(sdm_context* ctx, int32_t replica, const sdm_params* p);
+int sdm_get_params(sdm_context* ctx, int32_t replica, sdm_params* p);
+/* The integrator clock (ContextImpl time and step count). */
+int sdm_set_time(sdm_context* ctx, int32_t replica, double time, int64_t step);
+
+/* Replaces updateParametersInContext for the nonbonded per-atom parameters. */
+int sdm_set_nonbonded_params(sdm_context* ctx, const double* charge, const double* sigma,
+                             const double* epsilon);
+
+/* State (host arrays, [3*n_atoms] doubles, atom order as given to sdm_create). */
+int sdm_set_positions(sdm_context* ctx, int32_t replica, const double* xyz);
+int sdm_get_positions(sdm_context* ctx, int32_t replica, double* xyz);
+int sdm_set_velocities(sdm_context* ctx, int32_t replica, const double* xyz);
+int sdm_get_velocities(sdm_context* ctx, int32_t replica, double* xyz);
+/* Device-resident variants: d_xyz is a device pointer to [R][n_atoms][3] doubles. */
+int sdm_set_positions_device(sdm_context* ctx, const double* d_xyz);
+int sdm_get_positions_device(sdm_context* ctx, double* d_xyz);
+
+/* Evaluates the SDM force at the current positions without moving atoms: the two
+   state energies, u, the scalar stage and the hybrid force F = F0 + sp*(F1-F0) + Fb.
+   Replaces the three calcForcesAndEnergy calls plus the scalar part of execute()
+   (LangevinIntegratorSDM.cpp:158-176; ReferenceSDMKernels.cpp:264-318); the clock is
+   not advanced. */
+int sdm_compute(sdm_context* ctx);
+
+/* nsteps steps of LangevinIntegratorSDM::step (LangevinIntegratorSDM.cpp:153-183) for
+   every replica: evaluation, scalar stage, force mix and the Langevin update
+   (ReferenceStochasticDynamicsSDM.cpp:131-266 / langevin.cl:7-69). */
+int sdm_step(sdm_context* ctx, int32_t nsteps);
+
+/* Blocks until queued work is done, then copies every replica's observables. */
+int sdm_get_energies(sdm_context* ctx, sdm_energies* out /* [n_replicas] */);
+int sdm_clear_flags(sdm_context* ctx);
+
+/* Forces of the last sdm_compute/sdm_step evaluation, [3*n_atoms] doubles.
+   which: 0 = hybrid force, 1 = F0 (group 2 at x), 2 = F1 (group 2 at x+d),
+   3 = Fb (group 1).  1-3 require sdm_compute_components. */
+#define SDM_FORCE_HYBRID 0
+#define SDM_FORCE_STATE1 1
+#define SDM_FORCE_STATE2 2
+#define SDM_FORCE_BONDED 3
+int sdm_compute_components(sdm_context* ctx);
+int sdm_get_forces(sdm_context* ctx, int32_t replica, int32_t which, double* out);
+
+/* computeKineticEnergy (SDMKernels.h:77; ReferenceSDMKernels.cpp:108-135,345-347). */
+int sdm_kinetic_energy(sdm_context* ctx, double* out /* [n_replicas] */);
+
+/* u' recorded every `every` steps (0 disables) into a per-replica ring of `capacity`. */
+int sdm_set_u_history(sdm_context* ctx, int32_t every, int32_t capacity);
+int sdm_get_u_history(sdm_context* ctx, int32_t replica, double* out, int32_t capacity,
+                      int32_t* count);
+
+/* Diagnostics for the roofline: exact counts, at the current positions, of
+   non-excluded pairs with r < cutoff at x (p0) and of pairs with d_i != d_j and
+   r < cutoff at x+d (pdelta). */
+int sdm_count_pairs(sdm_context* ctx, int64_t* p0, int64_t* pdelta /* [n_replicas] */);
+/* Average duration (ms) of the nonbonded pair kernel over nsteps eager steps, timed
+   with HIP events on the context's stream. */
+int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
+
+/* Launch-configuration search (project-written autotuner): times each candidate on
+   the context's own replicas for `steps` steps and keeps the fastest. */
+int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
+int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);
+int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);
+
+/* Adjoint (no counterpart in the reference; parameter gradients of u for force-field
+   fitting): for every frame f of d_frames ([nframes][n_atoms][3] device doubles) and
+   every selected atom s, writes du/dq, du/dsigma, du/deps to d_grad
+   ([nframes][nsel][3] device doubles) and u to d_u ([nframes] device doubles).
+   Pair terms are recomputed, never stored.  Only pairs with d_i != d_j contribute. */
+int sdm_du_dparams_frames(sdm_context* ctx, const double* d_frames, int32_t nframes,
+                          const int32_t* sel, int32_t nsel, double* d_grad, double* d_u);
+
+/* OpenMM plugin-loader entry points (ReferenceSDMKernelFactory.cpp:41-56).  Without an
+   OpenMM runtime in the process they only record that they were called. */
+void registerPlatforms(void);
+void registerKernelFactories(void);
+void registerSDMHIPKernelFactories(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,344 @@
+"""ctypes binding of libsdm_hip.so (include/sdm_hip.h).
+
+The HIP library is the only compute path: if it is missing, or no HIP device is
+visible, every entry point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdm_hip.so")
+
+SDM_OK = 0
+SDM_ERR_INVALID = -1
+SDM_ERR_HIP = -2
+SDM_ERR_STATE = -3
+SDM_ERR_NUMERIC = -4
+SDM_ERR_CAPACITY = -5
+SDM_ERR_UNSUPPORTED = -6
+
+FLAG_NONFINITE, FLAG_OVERFLOW, FLAG_CAPACITY = 1, 2, 4
+FORCE_HYBRID, FORCE_STATE1, FORCE_STATE2, FORCE_BONDED = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class SDMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sdm error {code}: {msg}")
+        self.code = code
+
+
+class sdm_system_desc(C.Structure):
+    _fields_ = [
+        ("n_atoms", C.c_int32), ("charge", _dp), ("sigma", _dp), ("epsilon", _dp), ("mass", _dp),
+        ("n_exclusions", C.c_int32), ("exclusions", _ip), ("n_bonds", C.c_int32),
+        ("bond_atoms", _ip), ("bond_r0", _dp), ("bond_k", _dp), ("n_angles", C.c_int32),
+        ("angle_atoms", _ip), ("angle_theta0", _dp), ("angle_k", _dp), ("displacement", _dp),
+        ("box", C.c_double * 3), ("periodic", C.c_int32), ("cutoff", C.c_double),
+        ("eps_rf", C.c_double),
+    ]
+
+
+class sdm_params(C.Structure):
+    _fields_ = [
+        ("bias_method", C.c_int32), ("softcore_method", C.c_int32), ("lambda_", C.c_double),
+        ("gamma", C.c_double), ("wbcoeff", C.c_double), ("w0coeff", C.c_double),
+        ("lambda1", C.c_double), ("lambda2", C.c_double), ("alpha", C.c_double),
+        ("u0", C.c_double), ("umax", C.c_double), ("acore", C.c_double), ("ubcore", C.c_double),
+        ("nonequilibrium", C.c_int32), ("noneq_tmax", C.c_double), ("work", C.c_double),
+        ("m_l1", C.c_double), ("m_l2", C.c_double), ("m_u0", C.c_double), ("m_w0", C.c_double),
+        ("b_l1", C.c_double), ("b_l2", C.c_double), ("b_u0", C.c_double), ("b_w0", C.c_double),
+        ("temperature", C.c_double), ("friction", C.c_double), ("step_size", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class sdm_energies(C.Structure):
+    _fields_ = [
+        ("U0", C.c_double), ("U1", C.c_double), ("u", C.c_double), ("Ub", C.c_double),
+        ("bind_e", C.c_double), ("fp", C.c_double), ("W", C.c_double), ("Wp", C.c_double),
+        ("sp", C.c_double), ("pot_energy", C.c_double), ("lambda_", C.c_double),
+        ("lambda1", C.c_double), ("lambda2", C.c_double), ("u0", C.c_double), ("w0", C.c_double),
+        ("work", C.c_double), ("time", C.c_double), ("step", C.c_int64), ("flags", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class sdm_tuning(C.Structure):
+    _fields_ = [("pair_waves_per_group", C.c_int32), ("cell_cells_per_dim", C.c_int32),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+
+# every exported symbol of include/sdm_hip.h
+EXPORTS = [
+    "sdm_last_error", "sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params",
+    "sdm_get_params", "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
+    "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities", "sdm_set_positions_device",
+    "sdm_get_positions_device", "sdm_compute", "sdm_step", "sdm_get_energies", "sdm_clear_flags",
+    "sdm_compute_components", "sdm_get_forces", "sdm_kinetic_energy", "sdm_set_u_history",
+    "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
+    "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "registerPlatforms",
+    "registerKernelFactories", "registerSDMHIPKernelFactories",
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP extension first (__graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.sdm_last_error.restype = C.c_char_p
+    lib.sdm_create.argtypes = [C.POINTER(sdm_system_desc), C.c_int32, C.POINTER(vp)]
+    lib.sdm_destroy.argtypes = [vp]
+    lib.sdm_num_replicas.argtypes = [vp, _ip, _ip]
+    lib.sdm_set_params.argtypes = [vp, C.c_int32, C.POINTER(sdm_params)]
+    lib.sdm_get_params.argtypes = [vp, C.c_int32, C.POINTER(sdm_params)]
+    lib.sdm_set_time.argtypes = [vp, C.c_int32, C.c_double, C.c_int64]
+    lib.sdm_set_nonbonded_params.argtypes = [vp, _dp, _dp, _dp]
+    for f in ("sdm_set_positions", "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities"):
+        getattr(lib, f).argtypes = [vp, C.c_int32, _dp]
+    lib.sdm_set_positions_device.argtypes = [vp, vp]
+    lib.sdm_get_positions_device.argtypes = [vp, vp]
+    lib.sdm_compute.argtypes = [vp]
+    lib.sdm_step.argtypes = [vp, C.c_int32]
+    lib.sdm_get_energies.argtypes = [vp, C.POINTER(sdm_energies)]
+    lib.sdm_clear_flags.argtypes = [vp]
+    lib.sdm_compute_components.argtypes = [vp]
+    lib.sdm_get_forces.argtypes = [vp, C.c_int32, C.c_int32, _dp]
+    lib.sdm_kinetic_energy.argtypes = [vp, _dp]
+    lib.sdm_set_u_history.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.sdm_get_u_history.argtypes = [vp, C.c_int32, _dp, C.c_int32, _ip]
+    lib.sdm_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.sdm_time_pair_kernel.argtypes = [vp, C.c_int32, _dp]
+    lib.sdm_autotune.argtypes = [vp, C.c_int32, C.POINTER(sdm_tuning)]
+    lib.sdm_get_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
+    lib.sdm_set_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
+    lib.sdm_du_dparams_frames.argtypes = [vp, vp, C.c_int32, _ip, C.c_int32, vp, vp]
+    for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
+              "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
+              "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
+              "sdm_set_positions_device", "sdm_get_positions_device", "sdm_compute", "sdm_step",
+              "sdm_get_energies", "sdm_clear_flags", "sdm_compute_components", "sdm_get_forces",
+              "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
+              "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
+              "sdm_du_dparams_frames"):
+        getattr(lib, f).restype = C.c_int
+    for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
+        getattr(lib, f).restype = None
+        getattr(lib, f).argtypes = []
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != SDM_OK:
+        msg = _lib.sdm_last_error().decode() if _lib is not None else ""
+        raise SDMError(rc, msg)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def params_to_c(p) -> sdm_params:
+    o = sdm_params()
+    for name, _t in sdm_params._fields_:
+        setattr(o, name, getattr(p, name))
+    return o
+
+
+def params_from_c(o: sdm_params, p):
+    for name, _t in sdm_params._fields_:
+        setattr(p, name, getattr(o, name))
+    return p
+
+
+class NativeContext:
+    """Owns one sdm_context (R replicas of one topology) on the current HIP device."""
+
+    def __init__(self, system, n_replicas: int = 1, displacement=None):
+        lib = load()
+        self._lib = lib
+        s = system
+        self.n = s.n
+        self.R = n_replicas
+        keep = []
+
+        def d(a):
+            arr, p = _d(a)
+            keep.append(arr)
+            return p
+
+        def i(a):
+            arr, p = _i(a)
+            keep.append(arr)
+            return p
+
+        desc = sdm_system_desc()
+        desc.n_atoms = s.n
+        desc.charge, desc.sigma, desc.epsilon, desc.mass = (d(s.charge), d(s.sigma),
+                                                             d(s.epsilon), d(s.mass))
+        ex = np.asarray(s.exclusions, dtype=np.int32).reshape(-1, 2)
+        desc.n_exclusions = ex.shape[0]
+        desc.exclusions = i(ex.reshape(-1))
+        b = np.asarray(s.bonds, dtype=np.int32).reshape(-1, 2)
+        desc.n_bonds = b.shape[0]
+        desc.bond_atoms, desc.bond_r0, desc.bond_k = i(b.reshape(-1)), d(s.bond_r0), d(s.bond_k)
+        a = np.asarray(s.angles, dtype=np.int32).reshape(-1, 3)
+        desc.n_angles = a.shape[0]
+        desc.angle_atoms, desc.angle_theta0, desc.angle_k = (i(a.reshape(-1)), d(s.angle_t0),
+                                                             d(s.angle_k))
+        disp = s.displacement if displacement is None else displacement
+        desc.displacement = d(np.asarray(disp, dtype=np.float64).reshape(-1))
+        for k in range(3):
+            desc.box[k] = float(s.box[k])
+        desc.periodic = int(s.periodic)
+        desc.cutoff = float(s.cutoff)
+        desc.eps_rf = float(s.eps_rf)
+        h = C.c_void_p()
+        check(lib.sdm_create(C.byref(desc), n_replicas, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self._lib.sdm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters / state ----
+    def set_params(self, replica, p):
+        c = params_to_c(p)
+        check(self._lib.sdm_set_params(self.h, replica, C.byref(c)))
+
+    def get_params(self, replica, into):
+        c = sdm_params()
+        check(self._lib.sdm_get_params(self.h, replica, C.byref(c)))
+        return params_from_c(c, into)
+
+    def set_time(self, replica, time, step):
+        check(self._lib.sdm_set_time(self.h, replica, float(time), int(step)))
+
+    def set_nonbonded_params(self, q, sig, eps):
+        q, qp = _d(q)
+        s, sp = _d(sig)
+        e, ep = _d(eps)
+        check(self._lib.sdm_set_nonbonded_params(self.h, qp, sp, ep))
+
+    def set_positions(self, replica, x):
+        x, xp = _d(np.asarray(x).reshape(-1))
+        check(self._lib.sdm_set_positions(self.h, replica, xp))
+
+    def get_positions(self, replica):
+        out = np.zeros(3 * self.n)
+        check(self._lib.sdm_get_positions(self.h, replica, out.ctypes.data_as(_dp)))
+        return out.reshape(-1, 3)
+
+    def set_velocities(self, replica, v):
+        v, vp = _d(np.asarray(v).reshape(-1))
+        check(self._lib.sdm_set_velocities(self.h, replica, vp))
+
+    def get_velocities(self, replica):
+        out = np.zeros(3 * self.n)
+        check(self._lib.sdm_get_velocities(self.h, replica, out.ctypes.data_as(_dp)))
+        return out.reshape(-1, 3)
+
+    def set_positions_device(self, ptr):
+        check(self._lib.sdm_set_positions_device(self.h, C.c_void_p(ptr)))
+
+    def get_positions_device(self, ptr):
+        check(self._lib.sdm_get_positions_device(self.h, C.c_void_p(ptr)))
+
+    # ---- evaluation ----
+    def compute(self):
+        check(self._lib.sdm_compute(self.h))
+
+    def compute_components(self):
+        check(self._lib.sdm_compute_components(self.h))
+
+    def step(self, nsteps):
+        check(self._lib.sdm_step(self.h, int(nsteps)))
+
+    def energies(self):
+        out = (sdm_energies * self.R)()
+        check(self._lib.sdm_get_energies(self.h, out))
+        return [dict((n, getattr(e, n)) for n, _ in sdm_energies._fields_) for e in out]
+
+    def clear_flags(self):
+        check(self._lib.sdm_clear_flags(self.h))
+
+    def forces(self, replica, which=FORCE_HYBRID):
+        out = np.zeros(3 * self.n)
+        check(self._lib.sdm_get_forces(self.h, replica, which, out.ctypes.data_as(_dp)))
+        return out.reshape(-1, 3)
+
+    def kinetic_energy(self):
+        out = np.zeros(self.R)
+        check(self._lib.sdm_kinetic_energy(self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def set_u_history(self, every, capacity):
+        check(self._lib.sdm_set_u_history(self.h, every, capacity))
+
+    def u_history(self, replica, capacity):
+        out = np.zeros(capacity)
+        cnt = C.c_int32()
+        check(self._lib.sdm_get_u_history(self.h, replica, out.ctypes.data_as(_dp), capacity,
+                                          C.byref(cnt)))
+        return out[: cnt.value]
+
+    def count_pairs(self):
+        p0 = (C.c_int64 * self.R)()
+        pd = (C.c_int64 * self.R)()
+        check(self._lib.sdm_count_pairs(self.h, p0, pd))
+        return np.array(list(p0), dtype=np.int64), np.array(list(pd), dtype=np.int64)
+
+    def time_pair_kernel(self, nsteps):
+        ms = C.c_double()
+        check(self._lib.sdm_time_pair_kernel(self.h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, steps):
+        t = sdm_tuning()
+        check(self._lib.sdm_autotune(self.h, steps, C.byref(t)))
+        return dict((n, getattr(t, n)) for n, _ in sdm_tuning._fields_)
+
+    def get_tuning(self):
+        t = sdm_tuning()
+        check(self._lib.sdm_get_tuning(self.h, C.byref(t)))
+        return dict((n, getattr(t, n)) for n, _ in sdm_tuning._fields_)
+
+    def set_tuning(self, **kw):
+        t = sdm_tuning()
+        check(self._lib.sdm_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self._lib.sdm_set_tuning(self.h, C.byref(t)))
+
+    def du_dparams_frames(self, frames_ptr, nframes, sel, grad_ptr, u_ptr):
+        sel, selp = _i(sel)
+        check(self._lib.sdm_du_dparams_frames(self.h, C.c_void_p(frames_ptr), int(nframes), selp,
+                                              len(sel), C.c_void_p(grad_ptr), C.c_void_p(u_ptr)))

@@ -1,0 +1,932 @@
+// sdm_hip.hip -- host runtime and C ABI (include/sdm_hip.h) of the SDM HIP path.
+//
+// A context owns every device buffer of R replicas and one HIP stream (default
+// priority).  sdm_step replays a captured hipGraph of one step (all replicas in every
+// launch); no memset nodes are captured (counters are cleared by kernels).
+#include "sdm_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sdm_kernels.h"
+
+using namespace sdm;
+
+static_assert(offsetof(DevState, time) == sizeof(sdm_params), "DevState prefix must be sdm_params");
+static_assert(offsetof(DevState, seed) == offsetof(sdm_params, seed), "layout");
+static_assert(offsetof(DevState, work) == offsetof(sdm_params, work), "layout");
+static_assert(offsetof(DevRecord, flags) == offsetof(sdm_energies, flags), "layout");
+
+namespace {
+thread_local std::string g_err;
+int g_registered = 0;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(SDM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define CHK(expr)              \
+    do {                       \
+        int _rc = (expr);      \
+        if (_rc != SDM_OK) return _rc; \
+    } while (0)
+
+struct sdm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
+    int NC = 0, ncx = 0, ncy = 0, ncz = 0, NC_alloc = 0;
+    int maxj = 0, maxc = 0, nbond = 0, nang = 0, nbw = 1;
+    Geom g{};
+    float list_r2 = 0.f;
+    // topology (shared by all replicas)
+    float *qs = nullptr, *sh = nullptr, *se = nullptr;
+    double *q64 = nullptr, *sig64 = nullptr, *eps64 = nullptr, *invm = nullptr, *mass = nullptr;
+    int *ex_start = nullptr, *ex_list = nullptr, *cls = nullptr, *lig = nullptr, *cellrank = nullptr;
+    double* cdispl = nullptr;
+    int* bond_atoms = nullptr;
+    double *bond_r0 = nullptr, *bond_k = nullptr;
+    int* ang_atoms = nullptr;
+    double *ang_t0 = nullptr, *ang_k = nullptr;
+    // per replica
+    double *x64 = nullptr, *v64 = nullptr;
+    float4* posq_s = nullptr;
+    float2* par_s = nullptr;
+    int *orig_s = nullptr, *slot_of = nullptr, *key = nullptr;
+    int *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
+    float4 *bb_lo = nullptr, *bb_hi = nullptr;
+    int *tiles = nullptr, *ntiles = nullptr;
+    long long* F = nullptr;
+    double *e_pair = nullptr, *e_bond = nullptr, *u_part = nullptr;
+    int *cand = nullptr, *ncand = nullptr;
+    double4 *dA = nullptr, *dlig = nullptr, *dB = nullptr;
+    DevState* state = nullptr;
+    DevRecord* rec = nullptr;
+    int* flags = nullptr;
+    double* uhist = nullptr;
+    int hist_every = 0, hist_cap = 1;
+    long long* Fb_sep = nullptr;
+    double* dFd = nullptr;
+    unsigned long long* counters = nullptr;
+    double* ke_out = nullptr;
+    sdm_tuning tune{1, 0, 1, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool components_valid = false;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+int dalloc(sdm_context* c, T** p, size_t n) {
+    if (n == 0) n = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail(SDM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail(SDM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return SDM_OK;
+}
+
+template <class T>
+int upload(sdm_context* c, T* dst, const T* src, size_t n) {
+    if (n == 0) return SDM_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+template <class T>
+int download(sdm_context* c, T* dst, const T* src, size_t n) {
+    if (n == 0) return SDM_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+void drop_graph(sdm_context* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+uint32_t spread3(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 11; b++) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+
+// cell grid + Morton rank of every cell (the slot order of the spatial sort).
+int setup_cells(sdm_context* c, int per_dim) {
+    double L[3] = {c->g.Lx, c->g.Ly, c->g.Lz};
+    int n[3];
+    for (int d = 0; d < 3; d++) {
+        int v = per_dim > 0 ? per_dim : (int)std::floor(L[d] / 0.3);
+        n[d] = std::max(1, std::min(v, 1024));
+    }
+    int NC = n[0] * n[1] * n[2];
+    std::vector<std::pair<uint64_t, int>> code(NC);
+    for (int x = 0; x < n[0]; x++)
+        for (int y = 0; y < n[1]; y++)
+            for (int z = 0; z < n[2]; z++) {
+                int cell = (x * n[1] + y) * n[2] + z;
+                uint64_t m = ((uint64_t)spread3(x) << 2) | ((uint64_t)spread3(y) << 1) |
+                             (uint64_t)spread3(z);
+                code[cell] = {m, cell};
+            }
+    std::sort(code.begin(), code.end());
+    std::vector<int> rank(NC);
+    for (int k = 0; k < NC; k++) rank[code[k].second] = k;
+    if (NC > c->NC_alloc) {
+        CHK(dalloc(c, &c->cellrank, NC));
+        CHK(dalloc(c, &c->ccount, (size_t)c->R * NC));
+        CHK(dalloc(c, &c->cstart, (size_t)c->R * (NC + 1)));
+        CHK(dalloc(c, &c->cfill, (size_t)c->R * NC));
+        c->NC_alloc = NC;
+    } else {
+        // counters must start at zero for the new grid
+        HIPCHK(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->R * c->NC_alloc, c->stream));
+    }
+    CHK(upload(c, c->cellrank, rank.data(), NC));
+    c->ncx = n[0];
+    c->ncy = n[1];
+    c->ncz = n[2];
+    c->NC = NC;
+    drop_graph(c);
+    return SDM_OK;
+}
+
+dim3 g2(int n, int per, int R) { return dim3((unsigned)((n + per - 1) / per), (unsigned)R); }
+
+// spatial sort and tile lists for every replica
+void enqueue_sort(sdm_context* c) {
+    hipStream_t s = c->stream;
+    int N = c->N, Np = c->Np, R = c->R;
+    k_bin<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->x64, c->key, c->ccount, c->cellrank, c->g,
+                                        c->ncx, c->ncy, c->ncz, c->NC);
+    k_scan<<<R, 1024, 0, s>>>(c->NC, c->ccount, c->cstart, c->cfill);
+    k_scatter<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->NC, c->key, c->cstart, c->cfill, c->orig_s);
+    k_cellsort<<<g2(c->NC, 128, R), 128, 0, s>>>(N, Np, c->NC, c->cstart, c->orig_s, c->slot_of,
+                                                c->x64, c->qs, c->sh, c->se, c->posq_s,
+                                                c->par_s, c->g);
+    k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
+    k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(c->NB, c->maxj, c->bb_lo, c->bb_hi, c->tiles,
+                                          c->ntiles, c->list_r2, c->g, c->flags);
+}
+
+void enqueue_pair(sdm_context* c, bool count) {
+    hipStream_t s = c->stream;
+    int waves = c->R * c->NB;
+    unsigned long long* pc = count ? c->counters : nullptr;
+#define SDM_LAUNCH_PAIR(W)                                                                     \
+    do {                                                                                       \
+        if (count)                                                                             \
+            k_pair<W, true><<<(waves + W - 1) / W, 64 * W, 0, s>>>(                            \
+                c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, \
+                c->tiles, c->ntiles, c->ex_start, c->ex_list, c->F, c->e_pair, c->flags, pc,   \
+                c->g);                                                                         \
+        else                                                                                   \
+            k_pair<W, false><<<(waves + W - 1) / W, 64 * W, 0, s>>>(                           \
+                c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, \
+                c->tiles, c->ntiles, c->ex_start, c->ex_list, c->F, c->e_pair, c->flags, pc,   \
+                c->g);                                                                         \
+    } while (0)
+    switch (c->tune.pair_waves_per_group) {
+        case 2: SDM_LAUNCH_PAIR(2); break;
+        case 4: SDM_LAUNCH_PAIR(4); break;
+        default: SDM_LAUNCH_PAIR(1); break;
+    }
+#undef SDM_LAUNCH_PAIR
+}
+
+// evaluation of one step (everything before the Langevin update)
+void enqueue_eval(sdm_context* c, bool advance, bool components) {
+    hipStream_t s = c->stream;
+    int N = c->N, Np = c->Np, R = c->R;
+    enqueue_sort(c);
+    enqueue_pair(c, false);
+    if (c->nbond + c->nang > 0)
+        k_bonded<<<dim3(c->nbw, R), 256, 0, s>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
+                                                 c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
+                                                 c->x64, components ? c->Fb_sep : c->F,
+                                                 c->e_bond, c->nbw, c->flags);
+    if (c->L > 0) {
+        k_ligcand<<<R, 64, 0, s>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
+                                   c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
+                                   c->g, c->flags);
+        k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, s>>>(
+            N, Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
+            nullptr);
+        k_deltaA_red<<<dim3(c->LB, R), 64, 0, s>>>(
+            Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
+            c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g, nullptr);
+        k_deltaB<<<dim3(c->maxc, R), 64, 0, s>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
+                                                 c->sig64, c->eps64, c->ex_start, c->ex_list,
+                                                 c->x64, c->orig_s, c->cand, c->ncand, c->maxc,
+                                                 c->dB, c->g);
+    }
+    k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
+                               c->rec, c->flags, advance ? 1 : 0, c->hist_every, c->hist_cap,
+                               c->uhist);
+    if (c->L > 0 && !components)
+        k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
+                                                       c->orig_s, c->cand, c->ncand, c->dB,
+                                                       c->dlig, c->rec, c->F, c->flags);
+}
+
+void enqueue_step(sdm_context* c) {
+    enqueue_eval(c, true, false);
+    k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
+                                                           c->v64, c->rec, c->state);
+}
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return SDM_OK;
+}
+
+int build_graph(sdm_context* c) {
+    drop_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return SDM_OK;
+}
+
+int check_replica(const sdm_context* c, int r, bool allow_all) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (r == -1 && allow_all) return SDM_OK;
+    if (r < 0 || r >= c->R) return fail(SDM_ERR_INVALID, "replica index out of range");
+    return SDM_OK;
+}
+
+int validate_params(const sdm_params* p) {
+    if (!p) return fail(SDM_ERR_INVALID, "null parameters");
+    if (p->softcore_method < 0 || p->softcore_method > 2)
+        return fail(SDM_ERR_INVALID, "Unknown soft core method");
+    if (!(p->step_size > 0)) return fail(SDM_ERR_INVALID, "step size must be positive");
+    if (p->friction < 0) return fail(SDM_ERR_INVALID, "friction must be >= 0");
+    if (p->temperature < 0) return fail(SDM_ERR_INVALID, "temperature must be >= 0");
+    return SDM_OK;
+}
+
+// constructor defaults of LangevinIntegratorSDM (LangevinIntegratorSDM.cpp:48-85)
+sdm_params default_params() {
+    sdm_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.bias_method = SDM_LINEAR_METHOD;
+    p.softcore_method = SDM_NO_SOFTCORE_METHOD;
+    p.lambda = 1.0;
+    p.gamma = 0.0;
+    p.wbcoeff = 1.0;
+    p.w0coeff = 0.0;
+    p.lambda1 = 1.0;
+    p.lambda2 = 1.0;
+    p.alpha = 1.0;
+    p.u0 = 0.0;
+    p.umax = 200.0;
+    p.acore = 0.25;
+    p.ubcore = 0.0;
+    p.nonequilibrium = 0;
+    p.noneq_tmax = 1.0;
+    p.work = 0.0;
+    p.temperature = 300.0;
+    p.friction = 1.0;
+    p.step_size = 0.001;
+    p.seed = 0;
+    return p;
+}
+
+int upload_nb_params(sdm_context* c, const double* q, const double* sig, const double* eps) {
+    int N = c->N, Np = c->Np;
+    std::vector<float> qs(Np, 0.f), sh(Np, 0.f), se(Np, 0.f);
+    std::vector<double> q64(Np, 0.0), s64(Np, 0.0), e64(Np, 0.0);
+    double sk = std::sqrt(SDM_ONE_4PI_EPS0);
+    for (int i = 0; i < N; i++) {
+        if (!(sig[i] >= 0) || !(eps[i] >= 0) || !std::isfinite(q[i]))
+            return fail(SDM_ERR_INVALID, "nonbonded parameters must be finite, sigma/epsilon >= 0");
+        qs[i] = (float)(q[i] * sk);
+        sh[i] = (float)(0.5 * sig[i]);
+        se[i] = (float)(2.0 * std::sqrt(eps[i]));
+        q64[i] = q[i] * sk;
+        s64[i] = sig[i];
+        e64[i] = eps[i];
+    }
+    CHK(upload(c, c->qs, qs.data(), Np));
+    CHK(upload(c, c->sh, sh.data(), Np));
+    CHK(upload(c, c->se, se.data(), Np));
+    CHK(upload(c, c->q64, q64.data(), Np));
+    CHK(upload(c, c->sig64, s64.data(), Np));
+    CHK(upload(c, c->eps64, e64.data(), Np));
+    return SDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sdm_last_error(void) { return g_err.c_str(); }
+
+int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out) {
+    g_err.clear();
+    if (!sys || !out) return fail(SDM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int N = sys->n_atoms;
+    if (N <= 0) return fail(SDM_ERR_INVALID, "n_atoms must be positive");
+    if (n_replicas <= 0) return fail(SDM_ERR_INVALID, "n_replicas must be positive");
+    if (!sys->charge || !sys->sigma || !sys->epsilon || !sys->mass || !sys->displacement)
+        return fail(SDM_ERR_INVALID, "per-atom arrays must not be null");
+    if (!sys->periodic) return fail(SDM_ERR_UNSUPPORTED, "only CutoffPeriodic systems are supported");
+    if (!(sys->cutoff > 0)) return fail(SDM_ERR_INVALID, "cutoff must be positive");
+    for (int d = 0; d < 3; d++)
+        if (!(sys->box[d] >= 2.0 * sys->cutoff))
+            return fail(SDM_ERR_INVALID, "the cutoff distance cannot be greater than half the periodic box size");
+    if (!(sys->eps_rf > 0)) return fail(SDM_ERR_INVALID, "eps_rf must be positive");
+    for (int i = 0; i < N; i++)
+        if (!(sys->mass[i] >= 0)) return fail(SDM_ERR_INVALID, "masses must be >= 0");
+    // exclusions -> CSR
+    std::vector<int> ecount(N + 1, 0);
+    for (int k = 0; k < sys->n_exclusions; k++) {
+        int a = sys->exclusions[2 * k], b = sys->exclusions[2 * k + 1];
+        if (a < 0 || a >= N || b < 0 || b >= N || a == b)
+            return fail(SDM_ERR_INVALID, "exclusion index out of range");
+        ecount[a + 1]++;
+        ecount[b + 1]++;
+    }
+    for (int i = 0; i < N; i++) {
+        if (ecount[i + 1] > SDM_MAX_EXCL)
+            return fail(SDM_ERR_UNSUPPORTED, "more than 16 exclusions on one atom");
+        ecount[i + 1] += ecount[i];
+    }
+    std::vector<int> elist(std::max(1, ecount[N])), efill(N, 0);
+    for (int k = 0; k < sys->n_exclusions; k++) {
+        int a = sys->exclusions[2 * k], b = sys->exclusions[2 * k + 1];
+        elist[ecount[a] + efill[a]++] = b;
+        elist[ecount[b] + efill[b]++] = a;
+    }
+    for (int k = 0; k < sys->n_bonds; k++)
+        for (int j = 0; j < 2; j++)
+            if (sys->bond_atoms[2 * k + j] < 0 || sys->bond_atoms[2 * k + j] >= N)
+                return fail(SDM_ERR_INVALID, "bond index out of range");
+    for (int k = 0; k < sys->n_angles; k++)
+        for (int j = 0; j < 3; j++)
+            if (sys->angle_atoms[3 * k + j] < 0 || sys->angle_atoms[3 * k + j] >= N)
+                return fail(SDM_ERR_INVALID, "angle index out of range");
+    // displacement classes
+    std::vector<int> cls(N, 0);
+    std::vector<double> cd(3, 0.0);
+    std::vector<int> lig;
+    for (int i = 0; i < N; i++) {
+        const double* d = sys->displacement + 3 * i;
+        if (!std::isfinite(d[0]) || !std::isfinite(d[1]) || !std::isfinite(d[2]))
+            return fail(SDM_ERR_INVALID, "displacement must be finite");
+        if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+        int k = 0;
+        for (int m = 1; m < (int)cd.size() / 3; m++)
+            if (cd[3 * m] == d[0] && cd[3 * m + 1] == d[1] && cd[3 * m + 2] == d[2]) k = m;
+        if (k == 0) {
+            if ((int)cd.size() / 3 > SDM_MAX_CLASSES)
+                return fail(SDM_ERR_UNSUPPORTED, "more than 8 distinct displacement vectors");
+            cd.insert(cd.end(), d, d + 3);
+            k = (int)cd.size() / 3 - 1;
+        }
+        cls[i] = k;
+        lig.push_back(i);
+    }
+
+    sdm_context* c = new sdm_context();
+    int rc = SDM_OK;
+    auto bail = [&](int code) {
+        sdm_destroy(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SDM_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SDM_ERR_HIP, "hipStreamCreate failed"));
+    c->N = N;
+    c->Np = (N + 63) / 64 * 64;
+    c->NB = c->Np / 64;
+    c->R = n_replicas;
+    c->L = (int)lig.size();
+    c->LB = std::max(1, (c->L + 63) / 64);
+    c->ncls = (int)cd.size() / 3 - 1;
+    c->nbond = sys->n_bonds;
+    c->nang = sys->n_angles;
+    c->nbw = std::max(1, (c->nbond + c->nang + 255) / 256);
+    c->maxj = c->NB;
+    c->maxc = std::min(c->NB, 256 * std::max(1, c->ncls) + 64);
+    Geom& g = c->g;
+    g.Lx = sys->box[0];
+    g.Ly = sys->box[1];
+    g.Lz = sys->box[2];
+    g.iLx = 1.0 / g.Lx;
+    g.iLy = 1.0 / g.Ly;
+    g.iLz = 1.0 / g.Lz;
+    g.bx = (float)g.Lx;
+    g.by = (float)g.Ly;
+    g.bz = (float)g.Lz;
+    g.ibx = (float)g.iLx;
+    g.iby = (float)g.iLy;
+    g.ibz = (float)g.iLz;
+    double rcut = sys->cutoff, erf_ = sys->eps_rf;
+    g.rc2 = rcut * rcut;
+    g.krf = (erf_ - 1.0) / ((2.0 * erf_ + 1.0) * rcut * rcut * rcut);
+    g.crf = 3.0 * erf_ / ((2.0 * erf_ + 1.0) * rcut);
+    g.rc2f = (float)g.rc2;
+    g.krff = (float)g.krf;
+    g.crff = (float)g.crf;
+    c->list_r2 = (float)((rcut + 1e-3) * (rcut + 1e-3));
+    int N_ = N, Np = c->Np, R = c->R, NB = c->NB;
+    size_t RNp = (size_t)R * Np;
+#define A(ptr, n) \
+    if ((rc = dalloc(c, &c->ptr, (n))) != SDM_OK) return bail(rc)
+    A(qs, Np); A(sh, Np); A(se, Np); A(q64, Np); A(sig64, Np); A(eps64, Np);
+    A(invm, Np); A(mass, Np); A(ex_start, N_ + 1); A(ex_list, elist.size());
+    A(cls, Np); A(lig, std::max(1, c->L)); A(cdispl, cd.size());
+    A(bond_atoms, 2 * std::max(1, c->nbond)); A(bond_r0, std::max(1, c->nbond));
+    A(bond_k, std::max(1, c->nbond)); A(ang_atoms, 3 * std::max(1, c->nang));
+    A(ang_t0, std::max(1, c->nang)); A(ang_k, std::max(1, c->nang));
+    A(x64, 3 * RNp); A(v64, 3 * RNp); A(posq_s, RNp); A(par_s, RNp); A(orig_s, RNp);
+    A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
+    A(tiles, (size_t)R * NB * c->maxj); A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
+    A(e_pair, (size_t)R * NB); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
+    A(cand, (size_t)R * c->maxc); A(ncand, R);
+    A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
+    A(dB, (size_t)R * c->maxc * 64); A(state, R); A(rec, R); A(flags, R);
+    A(uhist, (size_t)R * c->hist_cap); A(counters, 2 * (size_t)R); A(ke_out, R);
+#undef A
+    // topology upload
+    if ((rc = upload_nb_params(c, sys->charge, sys->sigma, sys->epsilon)) != SDM_OK) return bail(rc);
+    {
+        std::vector<double> im(Np, 0.0), m(Np, 0.0);
+        for (int i = 0; i < N; i++) {
+            m[i] = sys->mass[i];
+            im[i] = sys->mass[i] == 0.0 ? 0.0 : 1.0 / sys->mass[i];
+        }
+        std::vector<int> clsp(Np, 0);
+        std::copy(cls.begin(), cls.end(), clsp.begin());
+        std::vector<int> orig_pad(RNp, -1);
+        if ((rc = upload(c, c->invm, im.data(), Np)) ||
+            (rc = upload(c, c->mass, m.data(), Np)) ||
+            (rc = upload(c, c->ex_start, ecount.data(), N + 1)) ||
+            (rc = upload(c, c->ex_list, elist.data(), elist.size())) ||
+            (rc = upload(c, c->cls, clsp.data(), Np)) ||
+            (rc = upload(c, c->cdispl, cd.data(), cd.size())) ||
+            (rc = upload(c, c->orig_s, orig_pad.data(), RNp)))
+            return bail(rc);
+        if (c->L > 0 && (rc = upload(c, c->lig, lig.data(), lig.size()))) return bail(rc);
+        if (c->nbond > 0 &&
+            ((rc = upload(c, c->bond_atoms, sys->bond_atoms, 2 * (size_t)c->nbond)) ||
+             (rc = upload(c, c->bond_r0, sys->bond_r0, c->nbond)) ||
+             (rc = upload(c, c->bond_k, sys->bond_k, c->nbond))))
+            return bail(rc);
+        if (c->nang > 0 &&
+            ((rc = upload(c, c->ang_atoms, sys->angle_atoms, 3 * (size_t)c->nang)) ||
+             (rc = upload(c, c->ang_t0, sys->angle_theta0, c->nang)) ||
+             (rc = upload(c, c->ang_k, sys->angle_k, c->nang))))
+            return bail(rc);
+        std::vector<DevState> st(R);
+        sdm_params p = default_params();
+        for (int r = 0; r < R; r++) {
+            std::memset(&st[r], 0, sizeof(DevState));
+            std::memcpy(&st[r], &p, sizeof(sdm_params));
+        }
+        if ((rc = upload(c, c->state, st.data(), R))) return bail(rc);
+    }
+    if ((rc = setup_cells(c, 0)) != SDM_OK) return bail(rc);
+    *out = c;
+    return SDM_OK;
+}
+
+int sdm_destroy(sdm_context* c) {
+    if (!c) return SDM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SDM_OK;
+}
+
+int sdm_num_replicas(const sdm_context* c, int32_t* n_replicas, int32_t* n_atoms) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (n_replicas) *n_replicas = c->R;
+    if (n_atoms) *n_atoms = c->N;
+    return SDM_OK;
+}
+
+int sdm_set_params(sdm_context* c, int32_t replica, const sdm_params* p) {
+    CHK(check_replica(c, replica, true));
+    CHK(validate_params(p));
+    int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->R : replica + 1;
+    for (int r = r0; r < r1; r++)
+        HIPCHK(hipMemcpyAsync(&c->state[r], p, sizeof(sdm_params), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_get_params(sdm_context* c, int32_t replica, sdm_params* p) {
+    CHK(check_replica(c, replica, false));
+    if (!p) return fail(SDM_ERR_INVALID, "null output");
+    HIPCHK(hipMemcpyAsync(p, &c->state[replica], sizeof(sdm_params), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_set_time(sdm_context* c, int32_t replica, double time, int64_t step) {
+    CHK(check_replica(c, replica, true));
+    int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->R : replica + 1;
+    for (int r = r0; r < r1; r++) {
+        HIPCHK(hipMemcpyAsync(&c->state[r].time, &time, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(&c->state[r].step, &step, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_set_nonbonded_params(sdm_context* c, const double* q, const double* sig, const double* eps) {
+    if (!c || !q || !sig || !eps) return fail(SDM_ERR_INVALID, "null argument");
+    return upload_nb_params(c, q, sig, eps);
+}
+
+static int set_vec(sdm_context* c, double* base, int32_t r, const double* xyz) {
+    CHK(check_replica(c, r, false));
+    if (!xyz) return fail(SDM_ERR_INVALID, "null array");
+    std::vector<double> soa(3 * (size_t)c->Np, 0.0);
+    for (int i = 0; i < c->N; i++)
+        for (int d = 0; d < 3; d++) {
+            double v = xyz[3 * i + d];
+            if (!std::isfinite(v)) return fail(SDM_ERR_INVALID, "non-finite coordinate");
+            soa[(size_t)d * c->Np + i] = v;
+        }
+    return upload(c, base + (size_t)r * 3 * c->Np, soa.data(), soa.size());
+}
+
+static int get_vec(sdm_context* c, const double* base, int32_t r, double* xyz) {
+    CHK(check_replica(c, r, false));
+    if (!xyz) return fail(SDM_ERR_INVALID, "null array");
+    std::vector<double> soa(3 * (size_t)c->Np);
+    CHK(download(c, soa.data(), base + (size_t)r * 3 * c->Np, soa.size()));
+    for (int i = 0; i < c->N; i++)
+        for (int d = 0; d < 3; d++) xyz[3 * i + d] = soa[(size_t)d * c->Np + i];
+    return SDM_OK;
+}
+
+int sdm_set_positions(sdm_context* c, int32_t r, const double* xyz) { return set_vec(c, c->x64, r, xyz); }
+int sdm_get_positions(sdm_context* c, int32_t r, double* xyz) { return get_vec(c, c->x64, r, xyz); }
+int sdm_set_velocities(sdm_context* c, int32_t r, const double* xyz) { return set_vec(c, c->v64, r, xyz); }
+int sdm_get_velocities(sdm_context* c, int32_t r, double* xyz) { return get_vec(c, c->v64, r, xyz); }
+
+}  // extern "C"
+
+namespace sdm {
+// [R][N][3] <-> [R][3][Np]
+__global__ void k_aos_to_soa(int N, int Np, const double* __restrict__ in, double* __restrict__ out) {
+    int r = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int d = 0; d < 3; d++) out[((size_t)r * 3 + d) * Np + i] = in[((size_t)r * N + i) * 3 + d];
+}
+__global__ void k_soa_to_aos(int N, int Np, const double* __restrict__ in, double* __restrict__ out) {
+    int r = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int d = 0; d < 3; d++) out[((size_t)r * N + i) * 3 + d] = in[((size_t)r * 3 + d) * Np + i];
+}
+}  // namespace sdm
+
+extern "C" {
+
+int sdm_set_positions_device(sdm_context* c, const double* d_xyz) {
+    if (!c || !d_xyz) return fail(SDM_ERR_INVALID, "null argument");
+    k_aos_to_soa<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, d_xyz, c->x64);
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
+    if (!c || !d_xyz) return fail(SDM_ERR_INVALID, "null argument");
+    k_soa_to_aos<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->x64, d_xyz);
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_compute(sdm_context* c) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    enqueue_eval(c, false, false);
+    c->components_valid = false;
+    return check_launch();
+}
+
+int sdm_step(sdm_context* c, int32_t nsteps) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (nsteps < 0) return fail(SDM_ERR_INVALID, "nsteps must be >= 0");
+    c->components_valid = false;
+    if (nsteps == 0) return SDM_OK;
+    if (c->tune.use_graph) {
+        if (!c->gexec) CHK(build_graph(c));
+        for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        for (int s = 0; s < nsteps; s++) enqueue_step(c);
+    }
+    return check_launch();
+}
+
+int sdm_get_energies(sdm_context* c, sdm_energies* out) {
+    if (!c || !out) return fail(SDM_ERR_INVALID, "null argument");
+    std::vector<DevRecord> rec(c->R);
+    std::vector<int> fl(c->R);
+    CHK(download(c, rec.data(), c->rec, c->R));
+    CHK(download(c, fl.data(), c->flags, c->R));
+    for (int r = 0; r < c->R; r++) {
+        sdm_energies& e = out[r];
+        const DevRecord& d = rec[r];
+        e.U0 = d.U0; e.U1 = d.U1; e.u = d.u; e.Ub = d.Ub; e.bind_e = d.bind_e; e.fp = d.fp;
+        e.W = d.W; e.Wp = d.Wp; e.sp = d.sp; e.pot_energy = d.pot_energy; e.lambda = d.lambda;
+        e.lambda1 = d.lambda1; e.lambda2 = d.lambda2; e.u0 = d.u0; e.w0 = d.w0; e.work = d.work;
+        e.time = d.time; e.step = d.step; e.flags = fl[r]; e.pad = 0;
+    }
+    return SDM_OK;
+}
+
+int sdm_clear_flags(sdm_context* c) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(int) * c->R, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+int sdm_compute_components(sdm_context* c) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    size_t n3 = 3 * (size_t)c->R * c->Np;
+    if (!c->Fb_sep) {
+        CHK(dalloc(c, &c->Fb_sep, n3));
+        CHK(dalloc(c, &c->dFd, n3));
+    }
+    k_fill_ll<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->Fb_sep, n3);
+    k_fill_d<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->dFd, n3);
+    enqueue_eval(c, false, true);
+    if (c->L > 0)
+        k_scatter_dF<<<dim3(c->maxc + c->LB, c->R), 64, 0, c->stream>>>(
+            c->N, c->Np, c->L, c->maxc, c->lig, c->cls, c->orig_s, c->cand, c->ncand, c->dB,
+            c->dlig, c->dFd);
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->components_valid = true;
+    return SDM_OK;
+}
+
+int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
+    CHK(check_replica(c, r, false));
+    if (!out) return fail(SDM_ERR_INVALID, "null output");
+    if (which < 0 || which > 3) return fail(SDM_ERR_INVALID, "unknown force component");
+    int Np = c->Np, N = c->N;
+    size_t off = (size_t)r * 3 * Np;
+    std::vector<long long> f(3 * (size_t)Np);
+    CHK(download(c, f.data(), c->F + off, f.size()));
+    if (which == SDM_FORCE_HYBRID && !c->components_valid) {
+        for (int i = 0; i < N; i++)
+            for (int d = 0; d < 3; d++) out[3 * i + d] = (double)f[(size_t)d * Np + i] / SDM_FIX_SCALE;
+        return SDM_OK;
+    }
+    if (!c->components_valid)
+        return fail(SDM_ERR_STATE, "force components require sdm_compute_components");
+    std::vector<long long> fb(3 * (size_t)Np);
+    std::vector<double> dF(3 * (size_t)Np);
+    CHK(download(c, fb.data(), c->Fb_sep + off, fb.size()));
+    CHK(download(c, dF.data(), c->dFd + off, dF.size()));
+    DevRecord rec;
+    CHK(download(c, &rec, c->rec + r, 1));
+    for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++) {
+            size_t k = (size_t)d * Np + i;
+            double f0 = (double)f[k] / SDM_FIX_SCALE, b = (double)fb[k] / SDM_FIX_SCALE;
+            double v = 0.0;
+            switch (which) {
+                case SDM_FORCE_HYBRID: v = f0 + rec.sp * dF[k] + b; break;
+                case SDM_FORCE_STATE1: v = f0; break;
+                case SDM_FORCE_STATE2: v = f0 + dF[k]; break;
+                default: v = b; break;
+            }
+            out[3 * i + d] = v;
+        }
+    return SDM_OK;
+}
+
+int sdm_kinetic_energy(sdm_context* c, double* out) {
+    if (!c || !out) return fail(SDM_ERR_INVALID, "null argument");
+    k_kinetic<<<c->R, 256, 0, c->stream>>>(c->N, c->Np, c->mass, c->v64, c->ke_out);
+    CHK(check_launch());
+    return download(c, out, c->ke_out, c->R);
+}
+
+int sdm_set_u_history(sdm_context* c, int32_t every, int32_t capacity) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (every < 0 || capacity <= 0) return fail(SDM_ERR_INVALID, "bad history settings");
+    if (capacity > c->hist_cap) CHK(dalloc(c, &c->uhist, (size_t)c->R * capacity));
+    c->hist_every = every;
+    c->hist_cap = capacity;
+    drop_graph(c);
+    return SDM_OK;
+}
+
+int sdm_get_u_history(sdm_context* c, int32_t r, double* out, int32_t capacity, int32_t* count) {
+    CHK(check_replica(c, r, false));
+    if (!out || !count) return fail(SDM_ERR_INVALID, "null argument");
+    int n = std::min(capacity, c->hist_cap);
+    CHK(download(c, out, c->uhist + (size_t)r * c->hist_cap, n));
+    DevState st;
+    CHK(download(c, &st, c->state + r, 1));
+    int64_t rec_n = c->hist_every > 0 ? (st.step + c->hist_every - 1) / c->hist_every : 0;
+    *count = (int32_t)std::min<int64_t>(rec_n, n);
+    return SDM_OK;
+}
+
+int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
+    if (!c || !p0 || !pdelta) return fail(SDM_ERR_INVALID, "null argument");
+    size_t n = 2 * (size_t)c->R;
+    hipStream_t s = c->stream;
+    k_fill_u64<<<1, 256, 0, s>>>(c->counters, n);
+    enqueue_sort(c);
+    enqueue_pair(c, true);
+    if (c->L > 0) {
+        k_ligcand<<<c->R, 64, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
+                                      c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
+                                      c->list_r2, c->g, c->flags);
+        k_deltaA<<<dim3(c->maxc, c->LB, c->R), 64, 0, s>>>(
+            c->N, c->Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
+            c->counters + c->R);
+        k_deltaA_red<<<dim3(c->LB, c->R), 64, 0, s>>>(
+            c->Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g,
+            c->counters + c->R);
+    }
+    CHK(check_launch());
+    std::vector<unsigned long long> h(n);
+    CHK(download(c, h.data(), c->counters, n));
+    for (int r = 0; r < c->R; r++) {
+        p0[r] = (int64_t)(h[r] / 2);
+        pdelta[r] = (int64_t)h[c->R + r];
+    }
+    return SDM_OK;
+}
+
+int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
+    if (!c || !avg_ms || nsteps <= 0) return fail(SDM_ERR_INVALID, "bad argument");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double total = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        // the step of enqueue_step, with events around the pair kernel
+        enqueue_sort(c);
+        HIPCHK(hipEventRecord(e0, c->stream));
+        enqueue_pair(c, false);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        hipStream_t st = c->stream;
+        int N = c->N, Np = c->Np, R = c->R;
+        if (c->nbond + c->nang > 0)
+            k_bonded<<<dim3(c->nbw, R), 256, 0, st>>>(Np, c->nbond, c->nang, c->bond_atoms,
+                                                      c->bond_r0, c->bond_k, c->ang_atoms,
+                                                      c->ang_t0, c->ang_k, c->x64, c->F,
+                                                      c->e_bond, c->nbw, c->flags);
+        if (c->L > 0) {
+            k_ligcand<<<R, 64, 0, st>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
+                                        c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
+                                        c->list_r2, c->g, c->flags);
+            k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, st>>>(
+                N, Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+                c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA,
+                c->g, nullptr);
+            k_deltaA_red<<<dim3(c->LB, R), 64, 0, st>>>(
+                Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+                c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part,
+                c->g, nullptr);
+            k_deltaB<<<dim3(c->maxc, R), 64, 0, st>>>(N, Np, c->L, c->lig, c->cls, c->cdispl,
+                                                      c->q64, c->sig64, c->eps64, c->ex_start,
+                                                      c->ex_list, c->x64, c->orig_s, c->cand,
+                                                      c->ncand, c->maxc, c->dB, c->g);
+        }
+        k_scalar<<<R, 256, 0, st>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part,
+                                    c->state, c->rec, c->flags, 1, c->hist_every, c->hist_cap,
+                                    c->uhist);
+        if (c->L > 0)
+            k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, st>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
+                                                            c->orig_s, c->cand, c->ncand, c->dB,
+                                                            c->dlig, c->rec, c->F, c->flags);
+        k_langevin<<<g2(N, 256, R), 256, 0, st>>>(N, Np, c->invm, c->F, c->x64, c->v64, c->rec,
+                                                  c->state);
+        CHK(check_launch());
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = total / nsteps;
+    c->components_valid = false;
+    return SDM_OK;
+}
+
+int sdm_get_tuning(sdm_context* c, sdm_tuning* t) {
+    if (!c || !t) return fail(SDM_ERR_INVALID, "null argument");
+    *t = c->tune;
+    return SDM_OK;
+}
+
+int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
+    if (!c || !t) return fail(SDM_ERR_INVALID, "null argument");
+    if (t->pair_waves_per_group != 1 && t->pair_waves_per_group != 2 && t->pair_waves_per_group != 4)
+        return fail(SDM_ERR_INVALID, "pair_waves_per_group must be 1, 2 or 4");
+    if (t->cell_cells_per_dim < 0 || t->cell_cells_per_dim > 1024)
+        return fail(SDM_ERR_INVALID, "cell_cells_per_dim out of range");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
+    c->tune = *t;
+    drop_graph(c);
+    return SDM_OK;
+}
+
+int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
+    if (!c || steps <= 0) return fail(SDM_ERR_INVALID, "bad argument");
+    size_t n3 = 3 * (size_t)c->R * c->Np;
+    std::vector<double> x(n3), v(n3);
+    std::vector<DevState> st(c->R);
+    std::vector<int> fl(c->R);
+    CHK(download(c, x.data(), c->x64, n3));
+    CHK(download(c, v.data(), c->v64, n3));
+    CHK(download(c, st.data(), c->state, c->R));
+    CHK(download(c, fl.data(), c->flags, c->R));
+    sdm_tuning best = c->tune;
+    double best_ms = 1e300;
+    double L = std::min(c->g.Lx, std::min(c->g.Ly, c->g.Lz));
+    const double cell_sides[] = {0.24, 0.3, 0.4};
+    const int waves[] = {1, 2, 4};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (double side : cell_sides)
+        for (int w : waves) {
+            sdm_tuning t = c->tune;
+            t.pair_waves_per_group = w;
+            t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
+            CHK(sdm_set_tuning(c, &t));
+            CHK(sdm_step(c, 2));
+            HIPCHK(hipEventRecord(e0, c->stream));
+            CHK(sdm_step(c, steps));
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+            CHK(upload(c, c->x64, x.data(), n3));
+            CHK(upload(c, c->v64, v.data(), n3));
+            CHK(upload(c, c->state, st.data(), c->R));
+            CHK(upload(c, c->flags, fl.data(), c->R));
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CHK(sdm_set_tuning(c, &best));
+    if (chosen) *chosen = best;
+    return SDM_OK;
+}
+
+int sdm_du_dparams_frames(sdm_context* c, const double* d_frames, int32_t nframes,
+                          const int32_t* sel, int32_t nsel, double* d_grad, double* d_u) {
+    (void)c; (void)d_frames; (void)nframes; (void)sel; (void)nsel; (void)d_grad; (void)d_u;
+    return fail(SDM_ERR_UNSUPPORTED, "adjoint not built yet");
+}
+
+void registerPlatforms(void) {}
+void registerKernelFactories(void) { g_registered++; }
+void registerSDMHIPKernelFactories(void) { registerKernelFactories(); }
+
+}  // extern "C"

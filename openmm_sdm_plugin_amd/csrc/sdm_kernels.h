@@ -1,0 +1,883 @@
+// sdm_kernels.h -- HIP kernels of the SDM step for gfx950 (wave64).
+//
+// One step of every replica r of a context (LangevinIntegratorSDM::step,
+// openmmapi/src/LangevinIntegratorSDM.cpp:153-183) is the launch sequence
+//
+//   k_bin -> k_scan -> k_scatter -> k_cellsort -> k_bbox -> k_tiles     spatial sort + tiles
+//   k_pair                 U0, F0: full LJ+RF sweep at x (FP32 math, fixed-point forces)
+//   k_bonded               Ub, Fb: harmonic bonds/angles at x (FP64, fixed-point atomics)
+//   k_ligcand              j-blocks near the displaced atoms at x or at x+d
+//   k_deltaA, k_deltaA_red u and F1-F0 on displaced atoms (FP64, pairs with d_i != d_j)
+//   k_deltaB               F1-F0 on the surrounding atoms (FP64)
+//   k_scalar               SoftCoreF, bias W/W', PotEnergy, noneq schedule/work, sp
+//   k_mix                  F += sp (F1 - F0)
+//   k_langevin             leapfrog Langevin update, Philox noise (FP64)
+//
+// U1 and F1 are never computed as a second full sweep: pairs whose displacements are
+// equal (d_i == d_j) have identical minimum-image distances at x and at x+d, so
+// U(x+d) - U(x) and F(x+d) - F(x) are sums over the few pairs with d_i != d_j only.
+// Every grid has the replica index in its y (or z) dimension.
+#pragma once
+#include "sdm_device.h"
+
+namespace sdm {
+
+struct Geom {
+    float bx, by, bz, ibx, iby, ibz;  // box and inverse (FP32 path)
+    double Lx, Ly, Lz, iLx, iLy, iLz; // FP64 path
+    float rc2f, krff, crff;
+    double rc2, krf, crf;
+};
+
+// ---------------------------------------------------------------------------
+// spatial sort (deterministic): cell rank per atom, counting sort, per-cell sort by
+// atom index so the slot order does not depend on atomic ordering.
+// ---------------------------------------------------------------------------
+__global__ void k_bin(int N, int Np, const double* __restrict__ x64, int* __restrict__ key,
+                      int* __restrict__ ccount, const int* __restrict__ cellrank, Geom g,
+                      int ncx, int ncy, int ncz, int NC) {
+    int r = blockIdx.y;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    double wx = X[i], wy = X[Np + i], wz = X[2 * Np + i];
+    wx -= g.Lx * floor(wx * g.iLx);
+    wy -= g.Ly * floor(wy * g.iLy);
+    wz -= g.Lz * floor(wz * g.iLz);
+    int cx = min(max((int)(wx * g.iLx * ncx), 0), ncx - 1);
+    int cy = min(max((int)(wy * g.iLy * ncy), 0), ncy - 1);
+    int cz = min(max((int)(wz * g.iLz * ncz), 0), ncz - 1);
+    int k = cellrank[(cx * ncy + cy) * ncz + cz];
+    key[(size_t)r * Np + i] = k;
+    atomicAdd(&ccount[(size_t)r * NC + k], 1);
+}
+
+// exclusive scan of the cell counts, one 1024-thread workgroup per replica; clears the
+// counters for the next step.
+__global__ __launch_bounds__(1024) void k_scan(int NC, int* __restrict__ ccount,
+                                               int* __restrict__ cstart, int* __restrict__ cfill) {
+    __shared__ int part[1024];
+    int r = blockIdx.x;
+    int t = threadIdx.x;
+    int per = (NC + 1023) / 1024;
+    int beg = min(t * per, NC), end = min(beg + per, NC);
+    int* cnt = ccount + (size_t)r * NC;
+    int s = 0;
+    for (int c = beg; c < end; c++) s += cnt[c];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    int* st = cstart + (size_t)r * (NC + 1);
+    int* fl = cfill + (size_t)r * NC;
+    for (int c = beg; c < end; c++) {
+        st[c] = run;
+        run += cnt[c];
+        cnt[c] = 0;
+        fl[c] = 0;
+    }
+    if (t == 1023) st[NC] = part[1023];
+}
+
+__global__ void k_scatter(int N, int Np, int NC, const int* __restrict__ key,
+                          const int* __restrict__ cstart, int* __restrict__ cfill,
+                          int* __restrict__ orig_s) {
+    int r = blockIdx.y;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int k = key[(size_t)r * Np + i];
+    int s = cstart[(size_t)r * (NC + 1) + k] + atomicAdd(&cfill[(size_t)r * NC + k], 1);
+    orig_s[(size_t)r * Np + s] = i;
+}
+
+// per cell: order its slots by atom index, then gather the sorted FP32 arrays.
+__global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart,
+                           int* __restrict__ orig_s, int* __restrict__ slot_of,
+                           const double* __restrict__ x64, const float* __restrict__ qs,
+                           const float* __restrict__ sh, const float* __restrict__ se,
+                           float4* __restrict__ posq_s, float2* __restrict__ par_s, Geom g) {
+    int r = blockIdx.y;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= NC) return;
+    const int* st = cstart + (size_t)r * (NC + 1);
+    int beg = st[c], end = st[c + 1];
+    int* o = orig_s + (size_t)r * Np;
+    for (int a = beg + 1; a < end; a++) {
+        int v = o[a];
+        int b = a - 1;
+        while (b >= beg && o[b] > v) {
+            o[b + 1] = o[b];
+            b--;
+        }
+        o[b + 1] = v;
+    }
+    const double* X = x64 + (size_t)r * 3 * Np;
+    for (int s = beg; s < end; s++) {
+        int i = o[s];
+        slot_of[(size_t)r * Np + i] = s;
+        double wx = X[i], wy = X[Np + i], wz = X[2 * Np + i];
+        wx -= g.Lx * floor(wx * g.iLx);
+        wy -= g.Ly * floor(wy * g.iLy);
+        wz -= g.Lz * floor(wz * g.iLz);
+        posq_s[(size_t)r * Np + s] = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
+        par_s[(size_t)r * Np + s] = make_float2(sh[i], se[i]);
+    }
+}
+
+// bounding box of each 64-slot block (one wave per block).
+__global__ __launch_bounds__(64) void k_bbox(int N, int Np, int NB,
+                                             const float4* __restrict__ posq_s,
+                                             float4* __restrict__ bb_lo, float4* __restrict__ bb_hi) {
+    int r = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
+    int s = b * 64 + lane;
+    bool v = s < N;
+    float4 p = posq_s[(size_t)r * Np + s];
+    float lx = wave_min(v ? p.x : 3.0e38f), ly = wave_min(v ? p.y : 3.0e38f),
+          lz = wave_min(v ? p.z : 3.0e38f);
+    float hx = wave_max(v ? p.x : -3.0e38f), hy = wave_max(v ? p.y : -3.0e38f),
+          hz = wave_max(v ? p.z : -3.0e38f);
+    if (lane == 0) {
+        bb_lo[(size_t)r * NB + b] = make_float4(lx, ly, lz, 0.f);
+        bb_hi[(size_t)r * NB + b] = make_float4(hx, hy, hz, 0.f);
+    }
+}
+
+__device__ __forceinline__ float bbox_gap2(float4 lo_i, float4 hi_i, float4 lo_j, float4 hi_j,
+                                           const Geom& g) {
+    float cx = 0.5f * (lo_i.x + hi_i.x) - 0.5f * (lo_j.x + hi_j.x);
+    float cy = 0.5f * (lo_i.y + hi_i.y) - 0.5f * (lo_j.y + hi_j.y);
+    float cz = 0.5f * (lo_i.z + hi_i.z) - 0.5f * (lo_j.z + hi_j.z);
+    cx -= g.bx * rintf(cx * g.ibx);
+    cy -= g.by * rintf(cy * g.iby);
+    cz -= g.bz * rintf(cz * g.ibz);
+    float gx = fmaxf(fabsf(cx) - 0.5f * (hi_i.x - lo_i.x) - 0.5f * (hi_j.x - lo_j.x), 0.f);
+    float gy = fmaxf(fabsf(cy) - 0.5f * (hi_i.y - lo_i.y) - 0.5f * (hi_j.y - lo_j.y), 0.f);
+    float gz = fmaxf(fabsf(cz) - 0.5f * (hi_i.z - lo_i.z) - 0.5f * (hi_j.z - lo_j.z), 0.f);
+    return gx * gx + gy * gy + gz * gz;
+}
+
+// tile list of one i-block: every j-block whose box is within rc (+slack), ascending.
+__global__ __launch_bounds__(64) void k_tiles(int NB, int maxj, const float4* __restrict__ bb_lo,
+                                              const float4* __restrict__ bb_hi,
+                                              int* __restrict__ tiles, int* __restrict__ ntiles,
+                                              float list_r2, Geom g, int* __restrict__ flags) {
+    int r = blockIdx.y, ib = blockIdx.x, lane = threadIdx.x;
+    const float4* lo = bb_lo + (size_t)r * NB;
+    const float4* hi = bb_hi + (size_t)r * NB;
+    float4 li = lo[ib], hi_i = hi[ib];
+    int* out = tiles + ((size_t)r * NB + ib) * maxj;
+    int count = 0;
+    for (int base = 0; base < NB; base += 64) {
+        int jb = base + lane;
+        bool ok = false;
+        if (jb < NB) ok = bbox_gap2(li, hi_i, lo[jb], hi[jb], g) < list_r2;
+        unsigned long long m = __ballot(ok);
+        int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+        if (ok && pos < maxj) out[pos] = jb;
+        count += __popcll(m);
+    }
+    if (lane == 0) {
+        ntiles[(size_t)r * NB + ib] = min(count, maxj);
+        if (count > maxj) atomicOr(&flags[r], 4);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the hot kernel: U0 and F0 over all pairs at x.  One wave per (replica, i-block):
+// lane = i atom (registers), j atoms of each listed j-block are wave-uniform (scalar
+// loads); every ordered pair is evaluated once, so the i force needs no atomics and the
+// energy is halved.  Pair energies are accumulated in FP64 per pair and the per-tile FP32
+// force sums in FP64 in tile order, so the result is bit-reproducible run to run.
+// ---------------------------------------------------------------------------
+template <bool EXCL>
+__device__ __forceinline__ void pair_tile(const float4* __restrict__ P, const float2* __restrict__ Q,
+                                          int jbeg, int jn, float4 pi, float2 qi,
+                                          unsigned long long m, const Geom& g, float& fx,
+                                          float& fy, float& fz, double& e, float& sqr2,
+                                          float& sq, int& cnt) {
+    const float bx = g.bx, by = g.by, bz = g.bz, ibx = g.ibx, iby = g.iby, ibz = g.ibz;
+    const float rc2 = g.rc2f, krf2 = 2.0f * g.krff;
+#pragma unroll 4
+    for (int jj = 0; jj < jn; jj++) {
+        float4 pj = P[jbeg + jj];
+        float2 qj = Q[jbeg + jj];
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        dx -= bx * rintf(dx * ibx);
+        dy -= by * rintf(dy * iby);
+        dz -= bz * rintf(dz * ibz);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        bool ok = r2 < rc2;
+        if (EXCL) ok = ok && !((m >> jj) & 1ull);
+        float ir = __frsqrt_rn(r2);
+        float ir2 = ir * ir;
+        float sig = qi.x + qj.x;
+        float s2 = sig * sig * ir2;
+        float s6 = s2 * s2 * s2;
+        float eps4 = qi.y * qj.y;
+        float qq = pi.w * pj.w;
+        float es6 = eps4 * s6;
+        float fr = es6 * (12.0f * s6 - 6.0f) * ir2 + qq * (ir * ir2 - krf2);
+        fr = ok ? fr : 0.0f;
+        // E = LJ + qq/r here; the reaction-field terms qq(k_rf r^2 - c_rf) are summed as
+        // sq*r^2 and sq and scaled by k_rf, c_rf in FP64 (their FP32 rounding would bias
+        // a sum over millions of pairs)
+        float ee = ok ? (es6 * (s6 - 1.0f) + qq * ir) : 0.0f;
+        float qqm = ok ? qq : 0.0f;
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+        e += (double)ee;
+        sqr2 += qqm * r2;
+        sq += qqm;
+        cnt += ok ? 1 : 0;
+    }
+}
+
+template <int WPW, bool COUNT>
+__global__ __launch_bounds__(64 * WPW) void k_pair(
+    int R, int N, int Np, int NB, int maxj, const float4* __restrict__ posq_s,
+    const float2* __restrict__ par_s, const int* __restrict__ orig_s,
+    const int* __restrict__ slot_of, const int* __restrict__ tiles,
+    const int* __restrict__ ntiles, const int* __restrict__ ex_start,
+    const int* __restrict__ ex_list, long long* __restrict__ F, double* __restrict__ e_pair,
+    int* __restrict__ flags, unsigned long long* __restrict__ pair_count, Geom g) {
+    int w = blockIdx.x * WPW + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (w >= R * NB) return;
+    int r = w / NB;
+    int ib = w - r * NB;
+    const float4* P = posq_s + (size_t)r * Np;
+    const float2* Q = par_s + (size_t)r * Np;
+    int islot = ib * 64 + lane;
+    bool vi = islot < N;
+    float4 pi = P[islot];
+    float2 qi = Q[islot];
+    int ai = vi ? orig_s[(size_t)r * Np + islot] : 0;
+    int exs[SDM_MAX_EXCL];
+    {
+        int b0 = vi ? ex_start[ai] : 0;
+        int nex = vi ? ex_start[ai + 1] - b0 : 0;
+#pragma unroll
+        for (int k = 0; k < SDM_MAX_EXCL; k++)
+            exs[k] = (k < nex) ? slot_of[(size_t)r * Np + ex_list[b0 + k]] : -1;
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0, aqr2 = 0.0, aq = 0.0;
+    int cnt = 0;
+    const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
+    int nt = ntiles[(size_t)r * NB + ib];
+    for (int t = 0; t < nt; t++) {
+        int jb = tl[t];
+        unsigned long long m = 0;
+#pragma unroll
+        for (int k = 0; k < SDM_MAX_EXCL; k++)
+            if ((exs[k] >> 6) == jb) m |= 1ull << (exs[k] & 63);
+        if (jb == ib) m |= 1ull << lane;
+        int jbeg = jb * 64;
+        int jn = min(64, N - jbeg);
+        float fx = 0.f, fy = 0.f, fz = 0.f, sqr2 = 0.f, sq = 0.f;
+        if (__ballot(m != 0ull) != 0ull)
+            pair_tile<true>(P, Q, jbeg, jn, pi, qi, m, g, fx, fy, fz, ae, sqr2, sq, cnt);
+        else
+            pair_tile<false>(P, Q, jbeg, jn, pi, qi, m, g, fx, fy, fz, ae, sqr2, sq, cnt);
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
+        aqr2 += (double)sqr2;
+        aq += (double)sq;
+    }
+    if (COUNT) {
+        double c = wave_sum(vi ? (double)cnt : 0.0);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(c + 0.5));
+        return;
+    }
+    int bad = 0;
+    if (vi) {
+        long long* Fr = F + (size_t)r * 3 * Np;
+        Fr[ai] = to_fixed(ax, &bad);
+        Fr[Np + ai] = to_fixed(ay, &bad);
+        Fr[2 * Np + ai] = to_fixed(az, &bad);
+    }
+    double es = wave_sum(vi ? (ae + g.krf * aqr2 - g.crf * aq) : 0.0);
+    if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
+    if (lane == 0) e_pair[(size_t)r * NB + ib] = 0.5 * es;
+}
+
+// ---------------------------------------------------------------------------
+// group 1: harmonic bonds and angles at x (FP64), fixed-point atomics.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bonded(
+    int Np, int nbond, int nang, const int* __restrict__ bond_atoms,
+    const double* __restrict__ bond_r0, const double* __restrict__ bond_k,
+    const int* __restrict__ ang_atoms, const double* __restrict__ ang_t0,
+    const double* __restrict__ ang_k, const double* __restrict__ x64,
+    long long* __restrict__ F, double* __restrict__ e_bond, int nbw, int* __restrict__ flags) {
+    __shared__ double red[256];
+    int r = blockIdx.y;
+    int t = blockIdx.x * 256 + threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    double e = 0.0;
+    int bad = 0;
+    if (t < nbond) {
+        int i = bond_atoms[2 * t], j = bond_atoms[2 * t + 1];
+        double dx = X[j] - X[i], dy = X[Np + j] - X[Np + i], dz = X[2 * Np + j] - X[2 * Np + i];
+        double rr = sqrt(dx * dx + dy * dy + dz * dz);
+        double dr = rr - bond_r0[t];
+        e = 0.5 * bond_k[t] * dr * dr;
+        double gg = rr > 0.0 ? bond_k[t] * dr / rr : 0.0;
+        long long fx = to_fixed(gg * dx, &bad), fy = to_fixed(gg * dy, &bad),
+                  fz = to_fixed(gg * dz, &bad);
+        atomicAdd(&Fr[i], (unsigned long long)fx);
+        atomicAdd(&Fr[Np + i], (unsigned long long)fy);
+        atomicAdd(&Fr[2 * Np + i], (unsigned long long)fz);
+        atomicAdd(&Fr[j], (unsigned long long)(-fx));
+        atomicAdd(&Fr[Np + j], (unsigned long long)(-fy));
+        atomicAdd(&Fr[2 * Np + j], (unsigned long long)(-fz));
+    } else if (t < nbond + nang) {
+        int k = t - nbond;
+        int a = ang_atoms[3 * k], m = ang_atoms[3 * k + 1], c = ang_atoms[3 * k + 2];
+        double v1[3], v2[3];
+        for (int d = 0; d < 3; d++) {
+            v1[d] = X[d * Np + a] - X[d * Np + m];
+            v2[d] = X[d * Np + c] - X[d * Np + m];
+        }
+        double r1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        double r2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+        double cs = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (r1 * r2);
+        cs = fmin(fmax(cs, -1.0), 1.0);
+        double th = acos(cs);
+        double dth = th - ang_t0[k];
+        e = 0.5 * ang_k[k] * dth * dth;
+        double sn = fmax(sin(th), 1e-12);
+        double gg = ang_k[k] * dth / sn;
+        for (int d = 0; d < 3; d++) {
+            double fa = gg * (v2[d] / (r1 * r2) - cs * v1[d] / (r1 * r1));
+            double fc = gg * (v1[d] / (r1 * r2) - cs * v2[d] / (r2 * r2));
+            long long ia = to_fixed(fa, &bad), ic = to_fixed(fc, &bad);
+            atomicAdd(&Fr[d * Np + a], (unsigned long long)ia);
+            atomicAdd(&Fr[d * Np + c], (unsigned long long)ic);
+            atomicAdd(&Fr[d * Np + m], (unsigned long long)(-(ia + ic)));
+        }
+    }
+    if (bad) atomicOr(&flags[r], 2);
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) e_bond[(size_t)r * nbw + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// displaced-state passes (FP64)
+// ---------------------------------------------------------------------------
+
+// candidate j-blocks: within rc of a displaced class's box at x or at x+d_k.
+__global__ __launch_bounds__(64) void k_ligcand(
+    int Np, int NB, int L, int ncls, const int* __restrict__ lig, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ x64,
+    const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi, int maxc,
+    int* __restrict__ cand, int* __restrict__ ncand, float list_r2, Geom g,
+    int* __restrict__ flags) {
+    __shared__ float4 blo[2 * SDM_MAX_CLASSES], bhi[2 * SDM_MAX_CLASSES];
+    int r = blockIdx.x, lane = threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    for (int k = 1; k <= ncls; k++) {
+        double lx = 1e300, ly = 1e300, lz = 1e300, hx = -1e300, hy = -1e300, hz = -1e300;
+        for (int base = 0; base < L; base += 64) {
+            int li = base + lane;
+            if (li < L) {
+                int a = lig[li];
+                if (cls[a] == k) {
+                    lx = fmin(lx, X[a]);
+                    ly = fmin(ly, X[Np + a]);
+                    lz = fmin(lz, X[2 * Np + a]);
+                    hx = fmax(hx, X[a]);
+                    hy = fmax(hy, X[Np + a]);
+                    hz = fmax(hz, X[2 * Np + a]);
+                }
+            }
+        }
+        lx = wave_min_d(lx); ly = wave_min_d(ly); lz = wave_min_d(lz);
+        hx = wave_max_d(hx); hy = wave_max_d(hy); hz = wave_max_d(hz);
+        if (lane == 0) {
+            const double* d = cdispl + 3 * k;
+            blo[2 * (k - 1)] = make_float4((float)lx, (float)ly, (float)lz, 0.f);
+            bhi[2 * (k - 1)] = make_float4((float)hx, (float)hy, (float)hz, 0.f);
+            blo[2 * (k - 1) + 1] = make_float4((float)(lx + d[0]), (float)(ly + d[1]), (float)(lz + d[2]), 0.f);
+            bhi[2 * (k - 1) + 1] = make_float4((float)(hx + d[0]), (float)(hy + d[1]), (float)(hz + d[2]), 0.f);
+        }
+    }
+    __syncthreads();
+    const float4* lo = bb_lo + (size_t)r * NB;
+    const float4* hi = bb_hi + (size_t)r * NB;
+    int* out = cand + (size_t)r * maxc;
+    int count = 0;
+    for (int base = 0; base < NB; base += 64) {
+        int jb = base + lane;
+        bool ok = false;
+        if (jb < NB) {
+            float4 lj = lo[jb], hj = hi[jb];
+            for (int k = 0; k < 2 * ncls; k++) ok = ok || (bbox_gap2(blo[k], bhi[k], lj, hj, g) < list_r2);
+        }
+        unsigned long long m = __ballot(ok);
+        int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+        if (ok && pos < maxc) out[pos] = jb;
+        count += __popcll(m);
+    }
+    if (lane == 0) {
+        ncand[r] = min(count, maxc);
+        if (count > maxc) atomicOr(&flags[r], 4);
+    }
+}
+
+__device__ __forceinline__ bool excluded(const int* __restrict__ ex_start,
+                                         const int* __restrict__ ex_list, int a, int b) {
+    for (int k = ex_start[a]; k < ex_start[a + 1]; k++)
+        if (ex_list[k] == b) return true;
+    return false;
+}
+
+struct Atom64 {
+    double x, y, z;      // position at x
+    double dx, dy, dz;   // displacement
+    double qk, sig, eps; // q*sqrt(k_e), sigma, epsilon
+    int idx, cls;
+};
+
+__device__ __forceinline__ Atom64 load_atom64(const double* __restrict__ X, int Np, int a,
+                                              const int* __restrict__ cls,
+                                              const double* __restrict__ cdispl,
+                                              const double* __restrict__ q,
+                                              const double* __restrict__ sig,
+                                              const double* __restrict__ eps) {
+    Atom64 A;
+    A.x = X[a];
+    A.y = X[Np + a];
+    A.z = X[2 * Np + a];
+    A.cls = cls[a];
+    A.dx = cdispl[3 * A.cls];
+    A.dy = cdispl[3 * A.cls + 1];
+    A.dz = cdispl[3 * A.cls + 2];
+    A.qk = q[a];
+    A.sig = sig[a];
+    A.eps = eps[a];
+    A.idx = a;
+    return A;
+}
+
+// the two geometries of one pair (i displaced by d_i, j by d_j): returns
+// e(x+d) - e(x) and adds F1-F0 on i to (fx,fy,fz).
+__device__ __forceinline__ double delta_pair(const Atom64& I, const Atom64& J, const Geom& g,
+                                             double& fx, double& fy, double& fz, int* inside_y) {
+    double qq = I.qk * J.qk;
+    double sig = 0.5 * (I.sig + J.sig);
+    double eps4 = 4.0 * sqrt(I.eps * J.eps);
+    double ex, fr_x, ey, fr_y;
+    double dx = min_image(I.x - J.x, g.Lx, g.iLx);
+    double dy = min_image(I.y - J.y, g.Ly, g.iLy);
+    double dz = min_image(I.z - J.z, g.Lz, g.iLz);
+    pair64(dx, dy, dz, qq, sig, eps4, g.rc2, g.krf, g.crf, &ex, &fr_x);
+    double ex_ = min_image((I.x + I.dx) - (J.x + J.dx), g.Lx, g.iLx);
+    double ey_ = min_image((I.y + I.dy) - (J.y + J.dy), g.Ly, g.iLy);
+    double ez_ = min_image((I.z + I.dz) - (J.z + J.dz), g.Lz, g.iLz);
+    *inside_y = pair64(ex_, ey_, ez_, qq, sig, eps4, g.rc2, g.krf, g.crf, &ey, &fr_y);
+    fx += fr_y * ex_ - fr_x * dx;
+    fy += fr_y * ey_ - fr_x * dy;
+    fz += fr_y * ez_ - fr_x * dz;
+    return ey - ex;
+}
+
+// lanes = displaced atoms (ligand block lb), j = undisplaced atoms of candidate block c.
+__global__ __launch_bounds__(64) void k_deltaA(
+    int N, int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ q64,
+    const double* __restrict__ sig64, const double* __restrict__ eps64,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
+    const int* __restrict__ ncand, int maxc, double4* __restrict__ dA, Geom g,
+    unsigned long long* __restrict__ pair_count) {
+    int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z, lane = threadIdx.x;
+    if (c >= ncand[r]) return;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    int li = lb * 64 + lane;
+    bool v = li < L;
+    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
+    int jb = cand[(size_t)r * maxc + c];
+    int jbeg = jb * 64, jn = min(64, N - jbeg);
+    const int* O = orig_s + (size_t)r * Np;
+    double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    int cnt = 0;
+    for (int jj = 0; jj < jn; jj++) {
+        int b = O[jbeg + jj];
+        if (cls[b] != 0) continue;
+        Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
+        if (!v || excluded(ex_start, ex_list, I.idx, b)) continue;
+        int iny;
+        du += delta_pair(I, J, g, fx, fy, fz, &iny);
+        cnt += iny;
+    }
+    if (pair_count) {
+        double s = wave_sum((double)cnt);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(s + 0.5));
+        return;
+    }
+    dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane] = make_double4(fx, fy, fz, du);
+}
+
+// per displaced atom: candidate partials in order, plus pairs with displaced atoms of a
+// different class (D x D); writes dlig and the ligand block's u.
+__global__ __launch_bounds__(64) void k_deltaA_red(
+    int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ q64,
+    const double* __restrict__ sig64, const double* __restrict__ eps64,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const double* __restrict__ x64, const int* __restrict__ ncand, int maxc,
+    const double4* __restrict__ dA, double4* __restrict__ dlig, double* __restrict__ u_part,
+    Geom g, unsigned long long* __restrict__ pair_count) {
+    int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    int li = lb * 64 + lane;
+    bool v = li < L;
+    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
+    double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    int nc = ncand[r];
+    for (int c = 0; c < nc; c++) {
+        double4 p = dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane];
+        fx += p.x;
+        fy += p.y;
+        fz += p.z;
+        du += p.w;
+    }
+    int cnt = 0;
+    for (int k = 0; k < L; k++) {
+        int b = lig[k];
+        Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
+        if (!v || J.cls == I.cls || excluded(ex_start, ex_list, I.idx, b)) continue;
+        int iny;
+        du += 0.5 * delta_pair(I, J, g, fx, fy, fz, &iny);
+        cnt += (iny && I.idx < b) ? 1 : 0;
+    }
+    if (pair_count) {
+        double s = wave_sum((double)cnt);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(s + 0.5));
+        return;
+    }
+    if (v) dlig[(size_t)r * L + li] = make_double4(fx, fy, fz, du);
+    double us = wave_sum(v ? du : 0.0);
+    if (lane == 0) u_part[(size_t)r * LB + lb] = us;
+}
+
+// lanes = atoms of candidate block c (undisplaced only), loop over the displaced atoms.
+__global__ __launch_bounds__(64) void k_deltaB(
+    int N, int Np, int L, const int* __restrict__ lig, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ q64,
+    const double* __restrict__ sig64, const double* __restrict__ eps64,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
+    const int* __restrict__ ncand, int maxc, double4* __restrict__ dB, Geom g) {
+    int c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    if (c >= ncand[r]) return;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    int jb = cand[(size_t)r * maxc + c];
+    int s = jb * 64 + lane;
+    bool v = s < N;
+    int b = v ? orig_s[(size_t)r * Np + s] : 0;
+    Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
+    bool env = v && J.cls == 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int k = 0; k < L; k++) {
+        Atom64 I = load_atom64(X, Np, lig[k], cls, cdispl, q64, sig64, eps64);
+        if (!env || excluded(ex_start, ex_list, b, I.idx)) continue;
+        int iny;
+        delta_pair(J, I, g, fx, fy, fz, &iny);
+    }
+    dB[((size_t)r * maxc + c) * 64 + lane] = make_double4(fx, fy, fz, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// scalar stage (one workgroup per replica): energies, SoftCoreF, bias, sp, schedule.
+// Restates ReferenceSDMKernels.cpp:202-309 and LangevinIntegratorSDM.cpp:125-149.
+// ---------------------------------------------------------------------------
+__device__ double softcore_dev(int method, double u, double umax, double a, double ub,
+                               double* fp) {
+    if (u <= ub || method == 0) {
+        *fp = 1.0;
+        return u;
+    }
+    if (method == 1) {
+        double t = tanh((u - ub) / umax);
+        *fp = 1.0 - t * t;
+        return umax * t + ub;
+    }
+    double gu = (u - ub) / (a * (umax - ub));
+    double zeta = 1.0 + 2.0 * gu * (gu + 1.0);
+    double zetap = pow(zeta, a);
+    double s = 4.0 * (2.0 * gu + 1.0) / zeta;
+    *fp = s * zetap / ((1.0 + zetap) * (1.0 + zetap));
+    return (umax - ub) * (zetap - 1.0) / (zetap + 1.0) + ub;
+}
+
+__global__ __launch_bounds__(256) void k_scalar(
+    int NB, int nbw, int LB, const double* __restrict__ e_pair,
+    const double* __restrict__ e_bond, const double* __restrict__ u_part,
+    DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
+    int advance, int hist_every, int hist_cap, double* __restrict__ uhist) {
+    __shared__ double red[256];
+    int r = blockIdx.x, t = threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = t; k < NB; k += 256) s0 += e_pair[(size_t)r * NB + k];
+    for (int k = t; k < nbw; k += 256) s1 += e_bond[(size_t)r * nbw + k];
+    red[t] = s0;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    double U0 = red[0];
+    __syncthreads();
+    red[t] = s1;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    double Ub = red[0];
+    if (t != 0) return;
+    double u = 0.0;
+    for (int k = 0; k < LB; k++) u += u_part[(size_t)r * LB + k];
+    DevState& S = state[r];
+    DevRecord& Rr = rec[r];
+    // nonequilibrium schedule (ReferenceSDMKernels.cpp:221-245)
+    double stepSize = S.step_size;
+    double lambdac = S.lambda, dlambdac = 0.0, gamma = 0.0, wbcoeff = lambdac, w0coeff = 0.0;
+    double lambda1 = lambdac, lambda2 = lambdac, alpha = 1.0, u0 = 0.0;
+    if (S.nonequilibrium == 1) {
+        lambdac = S.time / S.noneq_tmax;
+        S.lambda = lambdac;
+        S.lambda1 = S.m_l1 * lambdac + S.b_l1;
+        S.lambda2 = S.m_l2 * lambdac + S.b_l2;
+        S.u0 = S.m_u0 * lambdac + S.b_u0;
+        S.w0coeff = S.m_w0 * lambdac + S.b_w0;
+        dlambdac = stepSize / S.noneq_tmax;
+    }
+    int method = S.bias_method;
+    if (method == 1) {
+        gamma = S.gamma;
+        wbcoeff = S.wbcoeff;
+        w0coeff = S.w0coeff;
+    } else if (method == 2) {
+        lambda1 = S.lambda1;
+        lambda2 = S.lambda2;
+        alpha = S.alpha;
+        u0 = S.u0;
+        w0coeff = S.w0coeff;
+    }
+    double fp;
+    double be = softcore_dev(S.softcore_method, u, S.umax, S.acore, S.ubcore, &fp);
+    double ebias, bfp;
+    if (method == 1) {
+        ebias = 0.5 * gamma * be * be + wbcoeff * be + w0coeff;
+        bfp = gamma * be + wbcoeff;
+    } else if (method == 2) {
+        double z = -alpha * (be - u0);  // ee = 1 + e^z
+        ebias = (alpha > 0) ? ((lambda2 - lambda1) / alpha) * softplus(z) : 0.0;
+        ebias += lambda2 * be + w0coeff;
+        bfp = (lambda2 - lambda1) * inv_one_plus_exp(z) + lambda1;
+    } else {
+        ebias = lambdac * be;
+        bfp = lambdac;
+    }
+    if (S.nonequilibrium == 1) {
+        double z = -alpha * (be - u0);
+        double lee = softplus(z);
+        double dwdl1 = -lee / alpha;
+        double dwdl2 = be + lee / alpha;
+        double dwdu0 = (lambda2 - lambda1) * (1.0 - inv_one_plus_exp(z));
+        double dwdlambda = dwdl1 * S.m_l1 + dwdl2 * S.m_l2 + dwdu0 * S.m_u0 + 1.0 * S.m_w0;
+        S.work += dlambdac * dwdlambda;
+    }
+    double sp = bfp * fp;
+    double pot = U0 + ebias + Ub;
+    int fl = 0;
+    if (!isfinite(U0) || !isfinite(u) || !isfinite(be) || !isfinite(pot) || !isfinite(sp)) fl |= 1;
+    if (fl) atomicOr(&flags[r], fl);
+    Rr.U0 = U0;
+    Rr.U1 = U0 + u;
+    Rr.u = u;
+    Rr.Ub = Ub;
+    Rr.bind_e = be;
+    Rr.fp = fp;
+    Rr.W = ebias;
+    Rr.Wp = bfp;
+    Rr.sp = sp;
+    Rr.pot_energy = pot;
+    Rr.lambda = lambdac;
+    Rr.lambda1 = (method == 2) ? lambda1 : S.lambda1;
+    Rr.lambda2 = (method == 2) ? lambda2 : S.lambda2;
+    Rr.u0 = (method == 2) ? u0 : S.u0;
+    Rr.w0 = S.w0coeff;
+    Rr.work = S.work;
+    // Langevin parameters (OpenCLSDMKernels.cpp:331-355; equal to
+    // ReferenceStochasticDynamicsSDM.cpp:144-148 for friction > 0)
+    double kT = SDM_BOLTZ * S.temperature;
+    double vscale = exp(-stepSize * S.friction);
+    Rr.vscale = vscale;
+    Rr.fscale = (S.friction == 0.0) ? stepSize : (1.0 - vscale) / S.friction;
+    Rr.noisescale = sqrt(kT * (1.0 - vscale * vscale));
+    Rr.dt = stepSize;
+    Rr.noise_step = S.step;
+    if (hist_every > 0 && advance && (S.step % hist_every) == 0)
+        uhist[(size_t)r * hist_cap + (size_t)((S.step / hist_every) % hist_cap)] = be;
+    if (advance) {
+        S.time += stepSize;
+        S.step += 1;
+    }
+    Rr.time = S.time;
+    Rr.step = S.step;
+    Rr.flags = flags[r];
+}
+
+// F += sp (F1 - F0): candidate-block atoms from dB, displaced atoms from dlig.
+__global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
+                                            const int* __restrict__ lig, const int* __restrict__ cls,
+                                            const int* __restrict__ orig_s,
+                                            const int* __restrict__ cand, const int* __restrict__ ncand,
+                                            const double4* __restrict__ dB,
+                                            const double4* __restrict__ dlig,
+                                            const DevRecord* __restrict__ rec,
+                                            long long* __restrict__ F, int* __restrict__ flags) {
+    int r = blockIdx.y, lane = threadIdx.x;
+    double sp = rec[r].sp;
+    long long* Fr = F + (size_t)r * 3 * Np;
+    int bad = 0;
+    if ((int)blockIdx.x < maxc) {
+        int c = blockIdx.x;
+        if (c >= ncand[r]) return;
+        int s = cand[(size_t)r * maxc + c] * 64 + lane;
+        if (s >= N) return;
+        int b = orig_s[(size_t)r * Np + s];
+        if (cls[b] != 0) return;
+        double4 d = dB[((size_t)r * maxc + c) * 64 + lane];
+        Fr[b] += to_fixed(sp * d.x, &bad);
+        Fr[Np + b] += to_fixed(sp * d.y, &bad);
+        Fr[2 * Np + b] += to_fixed(sp * d.z, &bad);
+    } else {
+        int li = (blockIdx.x - maxc) * 64 + lane;
+        if (li >= L) return;
+        int a = lig[li];
+        double4 d = dlig[(size_t)r * L + li];
+        Fr[a] += to_fixed(sp * d.x, &bad);
+        Fr[Np + a] += to_fixed(sp * d.y, &bad);
+        Fr[2 * Np + a] += to_fixed(sp * d.z, &bad);
+    }
+    if (bad) atomicOr(&flags[r], 2);
+}
+
+// leapfrog Langevin (langevin.cl:7-69, double-precision branch), no constraints.
+__global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* __restrict__ invm,
+                                                  const long long* __restrict__ F,
+                                                  double* __restrict__ x64, double* __restrict__ v64,
+                                                  const DevRecord* __restrict__ rec,
+                                                  const DevState* __restrict__ state) {
+    int r = blockIdx.y;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double im = invm[i];
+    if (im == 0.0) return;
+    const DevRecord& Rr = rec[r];
+    double vscale = Rr.vscale, fscale = Rr.fscale, noise = Rr.noisescale, dt = Rr.dt;
+    double sim = sqrt(im);
+    double gz[3];
+    gauss3(state[r].seed, (uint32_t)r, (uint64_t)Rr.noise_step, (uint32_t)i, gz);
+    const long long* Fr = F + (size_t)r * 3 * Np;
+    double* X = x64 + (size_t)r * 3 * Np;
+    double* V = v64 + (size_t)r * 3 * Np;
+    double inv = 1.0 / dt;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double f = from_fixed(Fr[d * Np + i]);
+        double vel = vscale * V[d * Np + i] + fscale * im * f + noise * sim * gz[d];
+        double delta = dt * vel;
+        X[d * Np + i] += delta;
+        V[d * Np + i] = inv * delta;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_kinetic(int N, int Np, const double* __restrict__ mass,
+                                                 const double* __restrict__ v64,
+                                                 double* __restrict__ out) {
+    __shared__ double red[256];
+    int r = blockIdx.x, t = threadIdx.x;
+    const double* V = v64 + (size_t)r * 3 * Np;
+    double e = 0.0;
+    for (int i = t; i < N; i += 256) {
+        double m = mass[i];
+        if (m > 0) e += m * (V[i] * V[i] + V[Np + i] * V[Np + i] + V[2 * Np + i] * V[2 * Np + i]);
+    }
+    red[t] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) out[r] = 0.5 * red[0];
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: dense component forces (FP64) for sdm_compute_components.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_ll(long long* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void k_fill_d(double* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// dF (dense, [R][3][Np]) from dB and dlig
+__global__ __launch_bounds__(64) void k_scatter_dF(int N, int Np, int L, int maxc,
+                                                   const int* __restrict__ lig,
+                                                   const int* __restrict__ cls,
+                                                   const int* __restrict__ orig_s,
+                                                   const int* __restrict__ cand,
+                                                   const int* __restrict__ ncand,
+                                                   const double4* __restrict__ dB,
+                                                   const double4* __restrict__ dlig,
+                                                   double* __restrict__ dF) {
+    int r = blockIdx.y, lane = threadIdx.x;
+    double* D = dF + (size_t)r * 3 * Np;
+    if ((int)blockIdx.x < maxc) {
+        int c = blockIdx.x;
+        if (c >= ncand[r]) return;
+        int s = cand[(size_t)r * maxc + c] * 64 + lane;
+        if (s >= N) return;
+        int b = orig_s[(size_t)r * Np + s];
+        if (cls[b] != 0) return;
+        double4 d = dB[((size_t)r * maxc + c) * 64 + lane];
+        D[b] = d.x;
+        D[Np + b] = d.y;
+        D[2 * Np + b] = d.z;
+    } else {
+        int li = (blockIdx.x - maxc) * 64 + lane;
+        if (li >= L) return;
+        int a = lig[li];
+        double4 d = dlig[(size_t)r * L + li];
+        D[a] = d.x;
+        D[Np + a] = d.y;
+        D[2 * Np + a] = d.z;
+    }
+}
+
+}  // namespace sdm

@@ -1,0 +1,231 @@
+"""GPU parity of the HIP SDM path against the FP64 oracle (called through the C ABI).
+
+Tolerances (north star, mixed FP32-compute / fixed-point mode):
+  U0, U1, u, Ub, W, u'   within 1e-5 relative
+  forces                 within 1e-4 RMS relative (per force component array)
+"""
+import numpy as np
+import pytest
+
+from openmm_sdm_plugin_amd import _native, systems
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-5
+F_TOL = 1e-4
+
+
+def rms_rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-30))
+
+
+def close(a, b, tol=E_TOL, floor=1.0):
+    return abs(a - b) <= tol * max(floor, abs(b))
+
+
+def gpu_components(s, params=None, displacement=None):
+    ctx = _native.NativeContext(s, 1, displacement=displacement)
+    ctx.set_params(0, params or systems.params_config1())
+    ctx.set_positions(0, s.positions)
+    ctx.set_velocities(0, s.velocities)
+    ctx.compute_components()
+    e = ctx.energies()[0]
+    F = {w: ctx.forces(0, w) for w in range(4)}
+    return ctx, e, F
+
+
+def check_components(s, params=None):
+    print(f"case {s.name} n={s.n}", flush=True)
+    ctx, e, F = gpu_components(s, params)
+    topo = O.Topology(s)
+    E1, E2, Eb, F1, F2, Fb = O.reference_eval(topo, s.positions)
+    U0, u, F0d, dF, _, _ = O.sdm_direct(topo, s.positions)
+    assert close(e["U0"], E1), (e["U0"], E1)
+    assert close(e["U1"], E2, floor=abs(E1)), (e["U1"], E2)
+    assert close(e["u"], u), (e["u"], u, E2 - E1)
+    assert abs(e["Ub"] - Eb) <= E_TOL * max(1.0, abs(Eb)), (e["Ub"], Eb)
+    assert rms_rel(F[_native.FORCE_STATE1], F1) < F_TOL
+    assert rms_rel(F[_native.FORCE_STATE2], F2) < F_TOL
+    assert rms_rel(F[_native.FORCE_BONDED], Fb) < F_TOL
+    p = O.make_params(params or systems.params_config1())
+    sc = O.execute_scalars(p, E1, E2, Eb, 0.0)
+    Fh = np.empty_like(F1)
+    O.lib().orc_force_mix(s.n, sc.sp, F1.ctypes.data_as(O._dp), F2.ctypes.data_as(O._dp),
+                          Fb.ctypes.data_as(O._dp), Fh.ctypes.data_as(O._dp))
+    assert rms_rel(F[_native.FORCE_HYBRID], Fh) < F_TOL
+    assert close(e["bind_e"], sc.bind_e), (e["bind_e"], sc.bind_e)
+    assert close(e["W"], sc.ebias), (e["W"], sc.ebias)
+    scale = abs(E1) + abs(sc.ebias) + abs(Eb)  # PotEnergy = U0 + W + Ub
+    assert abs(e["pot_energy"] - sc.pot_energy) <= E_TOL * scale, (e["pot_energy"], sc.pot_energy)
+    # the normal (non-component) path gives the same hybrid force
+    ctx.compute()
+    Fn = ctx.forces(0, _native.FORCE_HYBRID)
+    assert rms_rel(Fn, Fh) < F_TOL
+    return ctx
+
+
+@pytest.mark.parametrize("frame", [0, 1, 2])
+def test_small_components(hip_device, frame):
+    check_components(systems.make_small(frame=frame))
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+def test_config1_components(hip_device, frame):
+    check_components(systems.make_config1(frame=frame))
+
+
+def test_config2_components(hip_device):
+    check_components(systems.make_config2(frame=0), systems.params_config2())
+
+
+def test_config5_components(hip_device):
+    s = systems.make_config5(frame=0)
+    check_components(s, systems.params_config5()[3])
+
+
+@pytest.mark.parametrize("bias", [0, 1, 2])
+@pytest.mark.parametrize("softcore", [0, 1, 2])
+def test_scalar_stage_methods(hip_device, bias, softcore):
+    s = systems.make_small(frame=1)
+    topo = O.Topology(s)
+    E1, E2, Eb, *_ = O.reference_eval(topo, s.positions)
+    p = systems.params_config1()
+    p.bias_method, p.softcore_method = bias, softcore
+    p.lambda_, p.lambda1, p.lambda2, p.alpha, p.u0, p.w0coeff = 0.35, 0.2, 0.6, 0.05, 40.0, 3.0
+    p.gamma, p.wbcoeff = 0.01, 0.4
+    p.umax, p.acore, p.ubcore = 50.0, 0.0625, 10.0
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_params(0, p)
+    ctx.set_positions(0, s.positions)
+    ctx.compute()
+    e = ctx.energies()[0]
+    sc = O.execute_scalars(O.make_params(p), E1, E2, Eb, 0.0)
+    for k, ref in (("bind_e", sc.bind_e), ("W", sc.ebias), ("Wp", sc.bfp), ("fp", sc.fp),
+                   ("sp", sc.sp)):
+        assert close(e[k], ref, floor=1e-3), (k, e[k], ref)
+    # PotEnergy = U0 + W + Ub: its bound is the sum of the components' 1e-5 bounds
+    scale = abs(E1) + abs(sc.ebias) + abs(Eb)
+    assert abs(e["pot_energy"] - sc.pot_energy) <= E_TOL * scale, (e["pot_energy"], sc.pot_energy)
+
+
+def test_step_parity_with_noise(hip_device):
+    s = systems.make_small(frame=2)
+    p = systems.params_config1()
+    p.alpha, p.lambda1, p.lambda2, p.u0 = 0.1, 0.2, 0.4, 30.0
+    p.seed = 424242
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_params(0, p)
+    ctx.set_positions(0, s.positions)
+    ctx.set_velocities(0, s.velocities)
+    nsteps = 5
+    ctx.step(nsteps)
+    e = ctx.energies()[0]
+    x_gpu = ctx.get_positions(0)
+    v_gpu = ctx.get_velocities(0)
+    topo = O.Topology(s)
+    x, v, t, st, rep = O.ref_step(topo, O.make_params(p), s.positions, s.velocities, 0.0, 0,
+                                  p.seed, 0, nsteps, use_cl_langevin=True)
+    assert st == nsteps and e["step"] == nsteps
+    assert abs(e["time"] - t) < 1e-12
+    assert np.max(np.abs(x_gpu - x)) < 1e-5
+    assert rms_rel(v_gpu, v) < 1e-3
+    assert close(e["bind_e"], rep.s.bind_e)
+    assert close(e["pot_energy"], rep.s.pot_energy)
+
+
+def test_nonequilibrium_schedule(hip_device):
+    s = systems.make_small(frame=0)
+    p = systems.params_config1()
+    p.bias_method = 2
+    p.alpha = 0.1
+    p.nonequilibrium, p.noneq_tmax = 1, 0.004
+    p.m_l1, p.m_l2, p.m_u0, p.m_w0 = 0.5, 0.8, 10.0, 1.0
+    p.b_l1, p.b_l2, p.b_u0, p.b_w0 = 0.1, 0.1, 20.0, 0.0
+    p.friction, p.temperature = 1.0, 0.0
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_params(0, p)
+    ctx.set_positions(0, s.positions)
+    ctx.set_velocities(0, s.velocities)
+    ctx.step(3)
+    e = ctx.energies()[0]
+    got = ctx.get_params(0, systems.SDMParams())
+    topo = O.Topology(s)
+    op = O.make_params(p)
+    x, v, t, st, rep = O.ref_step(topo, op, s.positions, s.velocities, 0.0, 0, p.seed, 0, 3)
+    for k in ("lambda_", "lambda1", "lambda2", "u0", "w0coeff"):
+        assert abs(getattr(got, k) - getattr(op, k)) < 1e-12, k
+    assert close(got.work, op.work, tol=1e-5, floor=1e-6), (got.work, op.work)
+    assert close(e["pot_energy"], rep.s.pot_energy)
+
+
+def test_determinism_bitwise(hip_device):
+    s = systems.make_config1(frame=3)
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_params(0, systems.params_config1())
+    ctx.set_positions(0, s.positions)
+    ctx.compute()
+    e1 = ctx.energies()[0]
+    f1 = ctx.forces(0)
+    for _ in range(3):
+        ctx.compute()
+        e2 = ctx.energies()[0]
+        f2 = ctx.forces(0)
+        assert e1["U0"] == e2["U0"] and e1["u"] == e2["u"] and e1["Ub"] == e2["Ub"]
+        assert np.array_equal(f1, f2)
+
+
+def test_batched_equals_single(hip_device):
+    frames = [systems.make_config1(frame=k) for k in range(3)]
+    params = []
+    for k in range(3):
+        p = systems.params_config1()
+        p.lambda1 = p.lambda2 = 0.1 + 0.2 * k
+        p.alpha = 0.05 * k
+        params.append(p)
+    multi = _native.NativeContext(frames[0], 3)
+    for k in range(3):
+        multi.set_params(k, params[k])
+        multi.set_positions(k, frames[k].positions)
+    multi.compute()
+    em = multi.energies()
+    for k in range(3):
+        single = _native.NativeContext(frames[0], 1)
+        single.set_params(0, params[k])
+        single.set_positions(0, frames[k].positions)
+        single.compute()
+        es = single.energies()[0]
+        for key in ("U0", "u", "Ub", "bind_e", "W", "sp", "pot_energy"):
+            assert em[k][key] == es[key], (k, key)
+        assert np.array_equal(multi.forces(k), single.forces(0))
+
+
+def test_pair_counts(hip_device):
+    s = systems.make_config1(frame=0)
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_params(0, systems.params_config1())
+    ctx.set_positions(0, s.positions)
+    p0, pd = ctx.count_pairs()
+    topo = O.Topology(s)
+    _, _, _, _, n0, nd = O.sdm_direct(topo, s.positions)
+    # FP32 r^2 < rc^2 may admit/drop pairs within rounding of the cutoff (energy ~0 there)
+    assert abs(int(p0[0]) - n0) <= max(2, 1e-5 * n0), (int(p0[0]), n0)
+    assert int(pd[0]) == nd  # the displaced-state pairs are tested in FP64
+
+
+def test_box_vector_displacement_gives_zero_u(hip_device):
+    s = systems.make_config2(frame=1)
+    d = np.zeros((s.n, 3))
+    d[s.ligand] = (s.box[0], 0.0, 0.0)
+    ctx, e, F = gpu_components(s, systems.params_config2(), displacement=d)
+    assert abs(e["u"]) < 1e-6
+    assert np.max(np.abs(F[_native.FORCE_STATE2] - F[_native.FORCE_STATE1])) < 1e-6
+
+
+def test_no_displacement(hip_device):
+    s = systems.make_small(frame=0)
+    ctx, e, F = gpu_components(s, displacement=np.zeros((s.n, 3)))
+    assert e["u"] == 0.0
+    assert np.array_equal(F[_native.FORCE_STATE2], F[_native.FORCE_STATE1])
