@@ -69,6 +69,7 @@ struct sdm_context {
     int *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
     float4 *bb_lo = nullptr, *bb_hi = nullptr;
     int *tiles = nullptr, *ntiles = nullptr;
+    unsigned long long* tmask = nullptr;
     long long* F = nullptr;
     double *e_pair = nullptr, *e_bond = nullptr, *u_part = nullptr;
     int *cand = nullptr, *ncand = nullptr;
@@ -82,7 +83,7 @@ struct sdm_context {
     double* dFd = nullptr;
     unsigned long long* counters = nullptr;
     double* ke_out = nullptr;
-    sdm_tuning tune{1, 0, 1, 0};
+    sdm_tuning tune{1, 0, 1, 1};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool components_valid = false;
@@ -129,13 +130,37 @@ void drop_graph(sdm_context* c) {
     c->graph = nullptr;
 }
 
-uint32_t spread3(uint32_t v) {
-    uint32_t r = 0;
-    for (int b = 0; b < 11; b++) r |= ((v >> b) & 1u) << (3 * b);
-    return r;
+// Hilbert index of a cell (Skilling, "Programming the Hilbert curve", AIP Conf. Proc.
+// 707, 381 (2004): axes -> transposed index, then bit interleave).  Hilbert-ordered
+// 64-slot blocks are near-cubic, so tile boxes are tight.
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    uint32_t X[3] = {x, y, z};
+    uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        uint32_t P = Q - 1;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; i++) X[i] ^= t;
+    uint64_t idx = 0;
+    for (int b = bits - 1; b >= 0; b--)
+        for (int i = 0; i < 3; i++) idx = (idx << 1) | ((X[i] >> b) & 1u);
+    return idx;
 }
 
-// cell grid + Morton rank of every cell (the slot order of the spatial sort).
+// cell grid + Hilbert rank of every cell (the slot order of the spatial sort).
 int setup_cells(sdm_context* c, int per_dim) {
     double L[3] = {c->g.Lx, c->g.Ly, c->g.Lz};
     int n[3];
@@ -144,14 +169,14 @@ int setup_cells(sdm_context* c, int per_dim) {
         n[d] = std::max(1, std::min(v, 1024));
     }
     int NC = n[0] * n[1] * n[2];
+    int bits = 1;
+    while ((1 << bits) < std::max(n[0], std::max(n[1], n[2]))) bits++;
     std::vector<std::pair<uint64_t, int>> code(NC);
     for (int x = 0; x < n[0]; x++)
         for (int y = 0; y < n[1]; y++)
             for (int z = 0; z < n[2]; z++) {
                 int cell = (x * n[1] + y) * n[2] + z;
-                uint64_t m = ((uint64_t)spread3(x) << 2) | ((uint64_t)spread3(y) << 1) |
-                             (uint64_t)spread3(z);
-                code[cell] = {m, cell};
+                code[cell] = {hilbert3((uint32_t)x, (uint32_t)y, (uint32_t)z, bits), cell};
             }
     std::sort(code.begin(), code.end());
     std::vector<int> rank(NC);
@@ -187,34 +212,39 @@ void enqueue_sort(sdm_context* c) {
     k_scatter<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->NC, c->key, c->cstart, c->cfill, c->orig_s);
     k_cellsort<<<g2(c->NC, 128, R), 128, 0, s>>>(N, Np, c->NC, c->cstart, c->orig_s, c->slot_of,
                                                 c->x64, c->qs, c->sh, c->se, c->posq_s,
-                                                c->par_s, c->g);
+                                                c->par_s, c->F, c->g);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
-    k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(c->NB, c->maxj, c->bb_lo, c->bb_hi, c->tiles,
-                                          c->ntiles, c->list_r2, c->g, c->flags);
+    k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi,
+                                          c->tiles, c->tmask, c->ntiles, c->list_r2, c->g,
+                                          c->tune.pair_newton3, c->flags);
 }
 
 void enqueue_pair(sdm_context* c, bool count) {
     hipStream_t s = c->stream;
     int waves = c->R * c->NB;
     unsigned long long* pc = count ? c->counters : nullptr;
-#define SDM_LAUNCH_PAIR(W)                                                                     \
-    do {                                                                                       \
-        if (count)                                                                             \
-            k_pair<W, true><<<(waves + W - 1) / W, 64 * W, 0, s>>>(                            \
-                c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, \
-                c->tiles, c->ntiles, c->ex_start, c->ex_list, c->F, c->e_pair, c->flags, pc,   \
-                c->g);                                                                         \
-        else                                                                                   \
-            k_pair<W, false><<<(waves + W - 1) / W, 64 * W, 0, s>>>(                           \
-                c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, \
-                c->tiles, c->ntiles, c->ex_start, c->ex_list, c->F, c->e_pair, c->flags, pc,   \
-                c->g);                                                                         \
+#define SDM_PAIR_ARGS                                                                       \
+    c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, c->tiles, \
+        c->tmask, c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->F, c->e_pair,   \
+        c->flags, pc, c->g
+#define SDM_LAUNCH_PAIR(W)                                                                   \
+    do {                                                                                     \
+        dim3 gr((waves + W - 1) / W);                                                        \
+        if (newton) {                                                                        \
+            if (count) k_pair<W, true, true><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);           \
+            else k_pair<W, false, true><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);                \
+        } else {                                                                             \
+            if (count) k_pair<W, true, false><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);          \
+            else k_pair<W, false, false><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);               \
+        }                                                                                    \
     } while (0)
+    bool newton = c->tune.pair_newton3 != 0;
     switch (c->tune.pair_waves_per_group) {
         case 2: SDM_LAUNCH_PAIR(2); break;
         case 4: SDM_LAUNCH_PAIR(4); break;
         default: SDM_LAUNCH_PAIR(1); break;
     }
+#undef SDM_PAIR_ARGS
 #undef SDM_LAUNCH_PAIR
 }
 
@@ -227,8 +257,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components) {
     if (c->nbond + c->nang > 0)
         k_bonded<<<dim3(c->nbw, R), 256, 0, s>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
                                                  c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
-                                                 c->x64, components ? c->Fb_sep : c->F,
-                                                 c->e_bond, c->nbw, c->flags);
+                                                 c->x64, components ? nullptr : c->slot_of,
+                                                 components ? c->Fb_sep : c->F, c->e_bond,
+                                                 c->nbw, c->flags);
     if (c->L > 0) {
         k_ligcand<<<R, 64, 0, s>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                    c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
@@ -251,13 +282,13 @@ void enqueue_eval(sdm_context* c, bool advance, bool components) {
     if (c->L > 0 && !components)
         k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
                                                        c->orig_s, c->cand, c->ncand, c->dB,
-                                                       c->dlig, c->rec, c->F, c->flags);
+                                                       c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
 void enqueue_step(sdm_context* c) {
     enqueue_eval(c, true, false);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
-                                                           c->v64, c->rec, c->state);
+                                                           c->v64, c->rec, c->state, c->slot_of);
 }
 
 int check_launch() {
@@ -471,7 +502,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(ang_t0, std::max(1, c->nang)); A(ang_k, std::max(1, c->nang));
     A(x64, 3 * RNp); A(v64, 3 * RNp); A(posq_s, RNp); A(par_s, RNp); A(orig_s, RNp);
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
-    A(tiles, (size_t)R * NB * c->maxj); A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
+    A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
+    A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
     A(e_pair, (size_t)R * NB); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
     A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
@@ -706,10 +738,13 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
     int Np = c->Np, N = c->N;
     size_t off = (size_t)r * 3 * Np;
     std::vector<long long> f(3 * (size_t)Np);
+    std::vector<int> slot(Np);
     CHK(download(c, f.data(), c->F + off, f.size()));
+    CHK(download(c, slot.data(), c->slot_of + (size_t)r * Np, Np));
     if (which == SDM_FORCE_HYBRID && !c->components_valid) {
         for (int i = 0; i < N; i++)
-            for (int d = 0; d < 3; d++) out[3 * i + d] = (double)f[(size_t)d * Np + i] / SDM_FIX_SCALE;
+            for (int d = 0; d < 3; d++)
+                out[3 * i + d] = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE;
         return SDM_OK;
     }
     if (!c->components_valid)
@@ -723,7 +758,8 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
     for (int i = 0; i < N; i++)
         for (int d = 0; d < 3; d++) {
             size_t k = (size_t)d * Np + i;
-            double f0 = (double)f[k] / SDM_FIX_SCALE, b = (double)fb[k] / SDM_FIX_SCALE;
+            double f0 = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE;
+            double b = (double)fb[k] / SDM_FIX_SCALE;
             double v = 0.0;
             switch (which) {
                 case SDM_FORCE_HYBRID: v = f0 + rec.sp * dF[k] + b; break;
@@ -789,7 +825,7 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
     std::vector<unsigned long long> h(n);
     CHK(download(c, h.data(), c->counters, n));
     for (int r = 0; r < c->R; r++) {
-        p0[r] = (int64_t)(h[r] / 2);
+        p0[r] = (int64_t)(c->tune.pair_newton3 ? h[r] : h[r] / 2);
         pdelta[r] = (int64_t)h[c->R + r];
     }
     return SDM_OK;
@@ -812,8 +848,8 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
         if (c->nbond + c->nang > 0)
             k_bonded<<<dim3(c->nbw, R), 256, 0, st>>>(Np, c->nbond, c->nang, c->bond_atoms,
                                                       c->bond_r0, c->bond_k, c->ang_atoms,
-                                                      c->ang_t0, c->ang_k, c->x64, c->F,
-                                                      c->e_bond, c->nbw, c->flags);
+                                                      c->ang_t0, c->ang_k, c->x64, c->slot_of,
+                                                      c->F, c->e_bond, c->nbw, c->flags);
         if (c->L > 0) {
             k_ligcand<<<R, 64, 0, st>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
                                         c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
@@ -837,9 +873,9 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
         if (c->L > 0)
             k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, st>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
                                                             c->orig_s, c->cand, c->ncand, c->dB,
-                                                            c->dlig, c->rec, c->F, c->flags);
+                                                            c->dlig, c->rec, c->slot_of, c->F, c->flags);
         k_langevin<<<g2(N, 256, R), 256, 0, st>>>(N, Np, c->invm, c->F, c->x64, c->v64, c->rec,
-                                                  c->state);
+                                                  c->state, c->slot_of);
         CHK(check_launch());
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -891,9 +927,11 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     for (double side : cell_sides)
-        for (int w : waves) {
+        for (int w : waves)
+        for (int newton = 0; newton < 2; newton++) {
             sdm_tuning t = c->tune;
             t.pair_waves_per_group = w;
+            t.pair_newton3 = newton;
             t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
             CHK(sdm_set_tuning(c, &t));
             CHK(sdm_step(c, 2));

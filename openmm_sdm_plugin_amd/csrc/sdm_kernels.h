@@ -95,12 +95,14 @@ __global__ void k_scatter(int N, int Np, int NC, const int* __restrict__ key,
     orig_s[(size_t)r * Np + s] = i;
 }
 
-// per cell: order its slots by atom index, then gather the sorted FP32 arrays.
+// per cell: order its slots by atom index, then gather the sorted FP32 arrays and clear
+// the slot-ordered force accumulator.
 __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart,
                            int* __restrict__ orig_s, int* __restrict__ slot_of,
                            const double* __restrict__ x64, const float* __restrict__ qs,
                            const float* __restrict__ sh, const float* __restrict__ se,
-                           float4* __restrict__ posq_s, float2* __restrict__ par_s, Geom g) {
+                           float4* __restrict__ posq_s, float2* __restrict__ par_s,
+                           long long* __restrict__ F, Geom g) {
     int r = blockIdx.y;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= NC) return;
@@ -126,6 +128,10 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
         wz -= g.Lz * floor(wz * g.iLz);
         posq_s[(size_t)r * Np + s] = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
         par_s[(size_t)r * Np + s] = make_float2(sh[i], se[i]);
+        long long* Fr = F + (size_t)r * 3 * Np;  // forces are accumulated in slot order
+        Fr[s] = 0;
+        Fr[Np + s] = 0;
+        Fr[2 * Np + s] = 0;
     }
 }
 
@@ -161,25 +167,55 @@ __device__ __forceinline__ float bbox_gap2(float4 lo_i, float4 hi_i, float4 lo_j
     return gx * gx + gy * gy + gz * gz;
 }
 
-// tile list of one i-block: every j-block whose box is within rc (+slack), ascending.
-__global__ __launch_bounds__(64) void k_tiles(int NB, int maxj, const float4* __restrict__ bb_lo,
+// tile list of one i-block: every j-block (j >= i for the half list) whose box is within
+// rc (+slack) of the i-block's box, ascending, each with a 64-bit mask of the j atoms that
+// lie within rc of that box (atoms outside it cannot have an in-cutoff partner in the
+// i-block).
+__global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
+                                              const float4* __restrict__ posq_s,
+                                              const float4* __restrict__ bb_lo,
                                               const float4* __restrict__ bb_hi,
-                                              int* __restrict__ tiles, int* __restrict__ ntiles,
-                                              float list_r2, Geom g, int* __restrict__ flags) {
+                                              int* __restrict__ tiles,
+                                              unsigned long long* __restrict__ tmask,
+                                              int* __restrict__ ntiles, float list_r2, Geom g,
+                                              int half, int* __restrict__ flags) {
     int r = blockIdx.y, ib = blockIdx.x, lane = threadIdx.x;
     const float4* lo = bb_lo + (size_t)r * NB;
     const float4* hi = bb_hi + (size_t)r * NB;
+    const float4* P = posq_s + (size_t)r * Np;
     float4 li = lo[ib], hi_i = hi[ib];
+    float cx = 0.5f * (li.x + hi_i.x), cy = 0.5f * (li.y + hi_i.y), cz = 0.5f * (li.z + hi_i.z);
+    float hx = 0.5f * (hi_i.x - li.x), hy = 0.5f * (hi_i.y - li.y), hz = 0.5f * (hi_i.z - li.z);
     int* out = tiles + ((size_t)r * NB + ib) * maxj;
+    unsigned long long* mout = tmask + ((size_t)r * NB + ib) * maxj;
     int count = 0;
-    for (int base = 0; base < NB; base += 64) {
+    for (int base = half ? (ib & ~63) : 0; base < NB; base += 64) {
         int jb = base + lane;
         bool ok = false;
-        if (jb < NB) ok = bbox_gap2(li, hi_i, lo[jb], hi[jb], g) < list_r2;
+        if (jb < NB && (!half || jb >= ib)) ok = bbox_gap2(li, hi_i, lo[jb], hi[jb], g) < list_r2;
         unsigned long long m = __ballot(ok);
-        int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-        if (ok && pos < maxj) out[pos] = jb;
-        count += __popcll(m);
+        while (m) {
+            int k = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            int jb2 = base + k;
+            int s = jb2 * 64 + lane;
+            float4 p = P[s];
+            float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+            dx -= g.bx * rintf(dx * g.ibx);
+            dy -= g.by * rintf(dy * g.iby);
+            dz -= g.bz * rintf(dz * g.ibz);
+            float gx = fmaxf(fabsf(dx) - hx, 0.f), gy = fmaxf(fabsf(dy) - hy, 0.f),
+                  gz = fmaxf(fabsf(dz) - hz, 0.f);
+            bool in = (s < N) && (gx * gx + gy * gy + gz * gz < list_r2);
+            unsigned long long am = __ballot(in);
+            if (am != 0ull) {
+                if (lane == 0 && count < maxj) {
+                    out[count] = jb2;
+                    mout[count] = am;
+                }
+                count++;
+            }
+        }
     }
     if (lane == 0) {
         ntiles[(size_t)r * NB + ib] = min(count, maxj);
@@ -189,122 +225,280 @@ __global__ __launch_bounds__(64) void k_tiles(int NB, int maxj, const float4* __
 
 // ---------------------------------------------------------------------------
 // the hot kernel: U0 and F0 over all pairs at x.  One wave per (replica, i-block):
-// lane = i atom (registers), j atoms of each listed j-block are wave-uniform (scalar
-// loads); every ordered pair is evaluated once, so the i force needs no atomics and the
-// energy is halved.  Pair energies are accumulated in FP64 per pair and the per-tile FP32
-// force sums in FP64 in tile order, so the result is bit-reproducible run to run.
+// lane = i atom (registers), j atoms are wave-uniform (scalar loads) taken from each
+// tile's j mask.
+//   NEWTON = false: full tile list, every ordered pair once, i force stored (no atomics),
+//                   energy halved.
+//   NEWTON = true:  half tile list (j-block >= i-block, j > i inside the diagonal tile),
+//                   every pair once; the j forces of 8 j atoms at a time are summed over
+//                   the wave by a reduce-scatter and added with int64 atomics.
+// When a tile's two boxes fit in half a box length the periodic image is one shift for
+// the whole tile (applied to the i coordinates); otherwise each pair takes the minimum
+// image.  Pair energies are accumulated in FP64 per pair and the per-tile FP32 force sums
+// in FP64 in tile order; fixed-point atomics commute, so results are bit-reproducible.
 // ---------------------------------------------------------------------------
-template <bool EXCL>
-__device__ __forceinline__ void pair_tile(const float4* __restrict__ P, const float2* __restrict__ Q,
-                                          int jbeg, int jn, float4 pi, float2 qi,
-                                          unsigned long long m, const Geom& g, float& fx,
-                                          float& fy, float& fz, double& e, float& sqr2,
-                                          float& sq, int& cnt) {
-    const float bx = g.bx, by = g.by, bz = g.bz, ibx = g.ibx, iby = g.iby, ibz = g.ibz;
-    const float rc2 = g.rc2f, krf2 = 2.0f * g.krff;
-#pragma unroll 4
-    for (int jj = 0; jj < jn; jj++) {
-        float4 pj = P[jbeg + jj];
-        float2 qj = Q[jbeg + jj];
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        dx -= bx * rintf(dx * ibx);
-        dy -= by * rintf(dy * iby);
-        dz -= bz * rintf(dz * ibz);
-        float r2 = dx * dx + dy * dy + dz * dz;
-        bool ok = r2 < rc2;
-        if (EXCL) ok = ok && !((m >> jj) & 1ull);
-        float ir = __frsqrt_rn(r2);
-        float ir2 = ir * ir;
-        float sig = qi.x + qj.x;
-        float s2 = sig * sig * ir2;
-        float s6 = s2 * s2 * s2;
-        float eps4 = qi.y * qj.y;
-        float qq = pi.w * pj.w;
-        float es6 = eps4 * s6;
-        float fr = es6 * (12.0f * s6 - 6.0f) * ir2 + qq * (ir * ir2 - krf2);
-        fr = ok ? fr : 0.0f;
-        // E = LJ + qq/r here; the reaction-field terms qq(k_rf r^2 - c_rf) are summed as
-        // sq*r^2 and sq and scaled by k_rf, c_rf in FP64 (their FP32 rounding would bias
-        // a sum over millions of pairs)
-        float ee = ok ? (es6 * (s6 - 1.0f) + qq * ir) : 0.0f;
-        float qqm = ok ? qq : 0.0f;
-        fx += fr * dx;
-        fy += fr * dy;
-        fz += fr * dz;
-        e += (double)ee;
-        sqr2 += qqm * r2;
-        sq += qqm;
-        cnt += ok ? 1 : 0;
+struct PairAcc {
+    float fx, fy, fz;  // per-tile force on the lane's i atom
+    float sqr2, sq;    // per-tile sums of qq r^2 and qq (reaction-field energy terms)
+    double e;          // LJ + qq/r energy
+    int cnt;
+};
+
+// one pair (i on the lane, j uniform); returns F/r (0 outside the cutoff or excluded)
+template <bool MINIMG, bool EXCL>
+__device__ __forceinline__ float pair_one(float4 pi, float2 qi, float4 pj, float2 qj,
+                                          bool extra_ok, const Geom& g, PairAcc& a,
+                                          float& dx, float& dy, float& dz) {
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+    if (MINIMG) {
+        dx -= g.bx * rintf(dx * g.ibx);
+        dy -= g.by * rintf(dy * g.iby);
+        dz -= g.bz * rintf(dz * g.ibz);
+    }
+    float r2 = dx * dx + dy * dy + dz * dz;
+    bool ok = r2 < g.rc2f;
+    if (EXCL) ok = ok && extra_ok;
+    float ir = __frsqrt_rn(r2);
+    float ir2 = ir * ir;
+    float sig = qi.x + qj.x;
+    float s2 = sig * sig * ir2;
+    float s6 = s2 * s2 * s2;
+    float es6 = (qi.y * qj.y) * s6;
+    float qq = pi.w * pj.w;
+    float fr = es6 * (12.0f * s6 - 6.0f) * ir2 + qq * (ir * ir2 - 2.0f * g.krff);
+    fr = ok ? fr : 0.0f;
+    // E = LJ + qq/r here; the reaction-field terms qq(k_rf r^2 - c_rf) are summed as
+    // sq*r^2 and sq and scaled by k_rf, c_rf in FP64 (their FP32 rounding would bias a
+    // sum over millions of pairs)
+    float ee = ok ? (es6 * (s6 - 1.0f) + qq * ir) : 0.0f;
+    float qqm = ok ? qq : 0.0f;
+    a.fx += fr * dx;
+    a.fy += fr * dy;
+    a.fz += fr * dz;
+    a.e += (double)ee;
+    a.sqr2 += qqm * r2;
+    a.sq += qqm;
+    a.cnt += ok ? 1 : 0;
+    return fr;
+}
+
+__device__ __forceinline__ int next_bit(unsigned long long& m) {
+    int j = m ? __ffsll((long long)m) - 1 : -1;
+    m &= m - 1ull;
+    return j;
+}
+
+// full evaluation: all j of one tile, four at a time so their scalar loads overlap
+template <bool MINIMG, bool EXCL>
+__device__ __forceinline__ void tile_full(const float4* __restrict__ P, const float2* __restrict__ Q,
+                                          int jbeg, unsigned long long am, float4 pi, float2 qi,
+                                          unsigned long long em, const Geom& g, PairAcc& a) {
+    float dx, dy, dz;
+    while (am) {
+        int j0 = next_bit(am), j1 = next_bit(am), j2 = next_bit(am), j3 = next_bit(am);
+        float4 p0 = P[jbeg + j0];
+        float2 q0 = Q[jbeg + j0];
+        float4 p1 = P[jbeg + max(j1, 0)];
+        float2 q1 = Q[jbeg + max(j1, 0)];
+        float4 p2 = P[jbeg + max(j2, 0)];
+        float2 q2 = Q[jbeg + max(j2, 0)];
+        float4 p3 = P[jbeg + max(j3, 0)];
+        float2 q3 = Q[jbeg + max(j3, 0)];
+        pair_one<MINIMG, EXCL>(pi, qi, p0, q0, !((em >> j0) & 1ull), g, a, dx, dy, dz);
+        if (j1 >= 0) pair_one<MINIMG, EXCL>(pi, qi, p1, q1, !((em >> j1) & 1ull), g, a, dx, dy, dz);
+        if (j2 >= 0) pair_one<MINIMG, EXCL>(pi, qi, p2, q2, !((em >> j2) & 1ull), g, a, dx, dy, dz);
+        if (j3 >= 0) pair_one<MINIMG, EXCL>(pi, qi, p3, q3, !((em >> j3) & 1ull), g, a, dx, dy, dz);
     }
 }
 
-template <int WPW, bool COUNT>
+// Sum v[24] (8 j atoms x 3 components, index 3k+c) over the 64 lanes so that lanes
+// 8k..8k+7 end up holding the three components of j atom k: three halving exchange
+// stages (lane bits 5, 4, 3) then a full sum over lane bits 2..0.
+__device__ __forceinline__ void reduce_scatter24(const float (&v)[24], int lane, float (&out)[3]) {
+    float w[12], x[6];
+    bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+#pragma unroll
+    for (int t = 0; t < 12; t++) {
+        float keep = b5 ? v[t + 12] : v[t];
+        float send = b5 ? v[t] : v[t + 12];
+        w[t] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        float keep = b4 ? w[t + 6] : w[t];
+        float send = b4 ? w[t] : w[t + 6];
+        x[t] = keep + __shfl_xor(send, 16, 64);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        float keep = b3 ? x[t + 3] : x[t];
+        float send = b3 ? x[t] : x[t + 3];
+        float y = keep + __shfl_xor(send, 8, 64);
+        y += __shfl_xor(y, 4, 64);
+        y += __shfl_xor(y, 2, 64);
+        y += __shfl_xor(y, 1, 64);
+        out[t] = y;
+    }
+}
+
+// Newton-3 evaluation of one tile: j atoms in groups of 8; their forces are scattered
+// to the slot-ordered fixed-point accumulator F (3 planes of Np).
+template <bool MINIMG, bool EXCL>
+__device__ __forceinline__ void tile_newton(const float4* __restrict__ P, const float2* __restrict__ Q,
+                                            int jbeg, unsigned long long am, float4 pi, float2 qi,
+                                            unsigned long long em, const Geom& g, PairAcc& a,
+                                            unsigned long long* __restrict__ Fr, int Np,
+                                            int lane, int& bad) {
+    while (am) {
+        int jj[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) jj[k] = next_bit(am);
+        float v[24];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[3 * k] = v[3 * k + 1] = v[3 * k + 2] = 0.f;
+            if (jj[k] >= 0) {
+                float4 pj = P[jbeg + jj[k]];
+                float2 qj = Q[jbeg + jj[k]];
+                float dx, dy, dz;
+                float fr = pair_one<MINIMG, EXCL>(pi, qi, pj, qj, !((em >> jj[k]) & 1ull), g, a,
+                                                  dx, dy, dz);
+                v[3 * k] = -fr * dx;
+                v[3 * k + 1] = -fr * dy;
+                v[3 * k + 2] = -fr * dz;
+            }
+        }
+        float f[3];
+        reduce_scatter24(v, lane, f);
+        int k = lane >> 3;
+        int mine = jj[0];
+#pragma unroll
+        for (int q = 1; q < 8; q++) mine = (k == q) ? jj[q] : mine;
+        if ((lane & 7) == 0 && mine >= 0) {
+            int s = jbeg + mine;
+            atomicAdd(&Fr[s], (unsigned long long)to_fixed((double)f[0], &bad));
+            atomicAdd(&Fr[Np + s], (unsigned long long)to_fixed((double)f[1], &bad));
+            atomicAdd(&Fr[2 * Np + s], (unsigned long long)to_fixed((double)f[2], &bad));
+        }
+    }
+}
+
+template <int WPW, bool COUNT, bool NEWTON>
 __global__ __launch_bounds__(64 * WPW) void k_pair(
     int R, int N, int Np, int NB, int maxj, const float4* __restrict__ posq_s,
     const float2* __restrict__ par_s, const int* __restrict__ orig_s,
     const int* __restrict__ slot_of, const int* __restrict__ tiles,
-    const int* __restrict__ ntiles, const int* __restrict__ ex_start,
-    const int* __restrict__ ex_list, long long* __restrict__ F, double* __restrict__ e_pair,
-    int* __restrict__ flags, unsigned long long* __restrict__ pair_count, Geom g) {
-    int w = blockIdx.x * WPW + (threadIdx.x >> 6);
+    const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
+    const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, long long* __restrict__ F,
+    double* __restrict__ e_pair, int* __restrict__ flags,
+    unsigned long long* __restrict__ pair_count, Geom g) {
+    __shared__ int exs_lds[WPW][SDM_MAX_EXCL][64];
+    // wave-uniform by construction; readfirstlane lets the compiler keep everything
+    // derived from it (replica, block, tile, j index) in scalar registers
+    int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int w = blockIdx.x * WPW + wv;
     int lane = threadIdx.x & 63;
     if (w >= R * NB) return;
     int r = w / NB;
     int ib = w - r * NB;
     const float4* P = posq_s + (size_t)r * Np;
     const float2* Q = par_s + (size_t)r * Np;
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
     int islot = ib * 64 + lane;
     bool vi = islot < N;
     float4 pi = P[islot];
     float2 qi = Q[islot];
-    int ai = vi ? orig_s[(size_t)r * Np + islot] : 0;
-    int exs[SDM_MAX_EXCL];
-    {
-        int b0 = vi ? ex_start[ai] : 0;
-        int nex = vi ? ex_start[ai + 1] - b0 : 0;
-#pragma unroll
-        for (int k = 0; k < SDM_MAX_EXCL; k++)
-            exs[k] = (k < nex) ? slot_of[(size_t)r * Np + ex_list[b0 + k]] : -1;
+    int nex = 0;
+    if (vi) {
+        int ai = orig_s[(size_t)r * Np + islot];
+        int b0 = ex_start[ai];
+        nex = ex_start[ai + 1] - b0;
+        for (int k = 0; k < nex; k++) exs_lds[wv][k][lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
     }
-    double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0, aqr2 = 0.0, aq = 0.0;
-    int cnt = 0;
+    float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
+    double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
+    PairAcc acc;
+    acc.e = 0.0;
+    acc.cnt = 0;
+    int bad = 0;
     const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
+    const unsigned long long* tm = tmask + ((size_t)r * NB + ib) * maxj;
     int nt = ntiles[(size_t)r * NB + ib];
     for (int t = 0; t < nt; t++) {
         int jb = tl[t];
-        unsigned long long m = 0;
-#pragma unroll
-        for (int k = 0; k < SDM_MAX_EXCL; k++)
-            if ((exs[k] >> 6) == jb) m |= 1ull << (exs[k] & 63);
-        if (jb == ib) m |= 1ull << lane;
+        unsigned long long am = tm[t];
+        unsigned long long em = 0;
+        for (int k = 0; k < nex; k++) {
+            int sl = exs_lds[wv][k][lane];
+            if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+        }
+        if (jb == ib) em |= NEWTON ? ((2ull << lane) - 1ull) : (1ull << lane);
+        // one periodic shift for the whole tile when both boxes fit in half a box
+        float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
+        float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
+        float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
+        float dcz = 0.5f * (bli.z + bhi.z - blj.z - bhj.z);
+        float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
+              sz = g.bz * rintf(dcz * g.ibz);
+        bool one_image =
+            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
+            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
+            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+        float4 ps = pi;
+        if (one_image) {
+            ps.x -= sx;
+            ps.y -= sy;
+            ps.z -= sz;
+        }
+        acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
+        bool ex = __ballot(em != 0ull) != 0ull;
         int jbeg = jb * 64;
-        int jn = min(64, N - jbeg);
-        float fx = 0.f, fy = 0.f, fz = 0.f, sqr2 = 0.f, sq = 0.f;
-        if (__ballot(m != 0ull) != 0ull)
-            pair_tile<true>(P, Q, jbeg, jn, pi, qi, m, g, fx, fy, fz, ae, sqr2, sq, cnt);
-        else
-            pair_tile<false>(P, Q, jbeg, jn, pi, qi, m, g, fx, fy, fz, ae, sqr2, sq, cnt);
-        ax += (double)fx;
-        ay += (double)fy;
-        az += (double)fz;
-        aqr2 += (double)sqr2;
-        aq += (double)sq;
+        if (NEWTON) {
+            if (one_image) {
+                if (ex) tile_newton<false, true>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+                else tile_newton<false, false>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+            } else {
+                if (ex) tile_newton<true, true>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+                else tile_newton<true, false>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+            }
+        } else {
+            if (one_image) {
+                if (ex) tile_full<false, true>(P, Q, jbeg, am, ps, qi, em, g, acc);
+                else tile_full<false, false>(P, Q, jbeg, am, ps, qi, em, g, acc);
+            } else {
+                if (ex) tile_full<true, true>(P, Q, jbeg, am, ps, qi, em, g, acc);
+                else tile_full<true, false>(P, Q, jbeg, am, ps, qi, em, g, acc);
+            }
+        }
+        ax += (double)acc.fx;
+        ay += (double)acc.fy;
+        az += (double)acc.fz;
+        aqr2 += (double)acc.sqr2;
+        aq += (double)acc.sq;
     }
     if (COUNT) {
-        double c = wave_sum(vi ? (double)cnt : 0.0);
+        double c = wave_sum(vi ? (double)acc.cnt : 0.0);
         if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(c + 0.5));
         return;
     }
-    int bad = 0;
     if (vi) {
-        long long* Fr = F + (size_t)r * 3 * Np;
-        Fr[ai] = to_fixed(ax, &bad);
-        Fr[Np + ai] = to_fixed(ay, &bad);
-        Fr[2 * Np + ai] = to_fixed(az, &bad);
+        long long fx = to_fixed(ax, &bad), fy = to_fixed(ay, &bad), fz = to_fixed(az, &bad);
+        if (NEWTON) {
+            atomicAdd(&Fr[islot], (unsigned long long)fx);
+            atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
+            atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
+        } else {
+            Fr[islot] = (unsigned long long)fx;
+            Fr[Np + islot] = (unsigned long long)fy;
+            Fr[2 * Np + islot] = (unsigned long long)fz;
+        }
     }
-    double es = wave_sum(vi ? (ae + g.krf * aqr2 - g.crf * aq) : 0.0);
+    double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
-    if (lane == 0) e_pair[(size_t)r * NB + ib] = 0.5 * es;
+    if (lane == 0) e_pair[(size_t)r * NB + ib] = NEWTON ? es : 0.5 * es;
 }
 
 // ---------------------------------------------------------------------------
@@ -315,16 +509,19 @@ __global__ __launch_bounds__(256) void k_bonded(
     const double* __restrict__ bond_r0, const double* __restrict__ bond_k,
     const int* __restrict__ ang_atoms, const double* __restrict__ ang_t0,
     const double* __restrict__ ang_k, const double* __restrict__ x64,
-    long long* __restrict__ F, double* __restrict__ e_bond, int nbw, int* __restrict__ flags) {
+    const int* __restrict__ slot_of, long long* __restrict__ F, double* __restrict__ e_bond,
+    int nbw, int* __restrict__ flags) {
     __shared__ double red[256];
     int r = blockIdx.y;
     int t = blockIdx.x * 256 + threadIdx.x;
     const double* X = x64 + (size_t)r * 3 * Np;
     unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    const int* S = slot_of ? slot_of + (size_t)r * Np : nullptr;
     double e = 0.0;
     int bad = 0;
     if (t < nbond) {
         int i = bond_atoms[2 * t], j = bond_atoms[2 * t + 1];
+        int si = S ? S[i] : i, sj = S ? S[j] : j;
         double dx = X[j] - X[i], dy = X[Np + j] - X[Np + i], dz = X[2 * Np + j] - X[2 * Np + i];
         double rr = sqrt(dx * dx + dy * dy + dz * dz);
         double dr = rr - bond_r0[t];
@@ -332,15 +529,16 @@ __global__ __launch_bounds__(256) void k_bonded(
         double gg = rr > 0.0 ? bond_k[t] * dr / rr : 0.0;
         long long fx = to_fixed(gg * dx, &bad), fy = to_fixed(gg * dy, &bad),
                   fz = to_fixed(gg * dz, &bad);
-        atomicAdd(&Fr[i], (unsigned long long)fx);
-        atomicAdd(&Fr[Np + i], (unsigned long long)fy);
-        atomicAdd(&Fr[2 * Np + i], (unsigned long long)fz);
-        atomicAdd(&Fr[j], (unsigned long long)(-fx));
-        atomicAdd(&Fr[Np + j], (unsigned long long)(-fy));
-        atomicAdd(&Fr[2 * Np + j], (unsigned long long)(-fz));
+        atomicAdd(&Fr[si], (unsigned long long)fx);
+        atomicAdd(&Fr[Np + si], (unsigned long long)fy);
+        atomicAdd(&Fr[2 * Np + si], (unsigned long long)fz);
+        atomicAdd(&Fr[sj], (unsigned long long)(-fx));
+        atomicAdd(&Fr[Np + sj], (unsigned long long)(-fy));
+        atomicAdd(&Fr[2 * Np + sj], (unsigned long long)(-fz));
     } else if (t < nbond + nang) {
         int k = t - nbond;
         int a = ang_atoms[3 * k], m = ang_atoms[3 * k + 1], c = ang_atoms[3 * k + 2];
+        int sa = S ? S[a] : a, sm = S ? S[m] : m, sc = S ? S[c] : c;
         double v1[3], v2[3];
         for (int d = 0; d < 3; d++) {
             v1[d] = X[d * Np + a] - X[d * Np + m];
@@ -359,9 +557,9 @@ __global__ __launch_bounds__(256) void k_bonded(
             double fa = gg * (v2[d] / (r1 * r2) - cs * v1[d] / (r1 * r1));
             double fc = gg * (v1[d] / (r1 * r2) - cs * v2[d] / (r2 * r2));
             long long ia = to_fixed(fa, &bad), ic = to_fixed(fc, &bad);
-            atomicAdd(&Fr[d * Np + a], (unsigned long long)ia);
-            atomicAdd(&Fr[d * Np + c], (unsigned long long)ic);
-            atomicAdd(&Fr[d * Np + m], (unsigned long long)(-(ia + ic)));
+            atomicAdd(&Fr[d * Np + sa], (unsigned long long)ia);
+            atomicAdd(&Fr[d * Np + sc], (unsigned long long)ic);
+            atomicAdd(&Fr[d * Np + sm], (unsigned long long)(-(ia + ic)));
         }
     }
     if (bad) atomicOr(&flags[r], 2);
@@ -744,7 +942,8 @@ __global__ __launch_bounds__(256) void k_scalar(
     Rr.flags = flags[r];
 }
 
-// F += sp (F1 - F0): candidate-block atoms from dB, displaced atoms from dlig.
+// F += sp (F1 - F0): candidate-block atoms from dB, displaced atoms from dlig (the
+// accumulator is in slot order).
 __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
                                             const int* __restrict__ lig, const int* __restrict__ cls,
                                             const int* __restrict__ orig_s,
@@ -752,10 +951,12 @@ __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
                                             const double4* __restrict__ dB,
                                             const double4* __restrict__ dlig,
                                             const DevRecord* __restrict__ rec,
+                                            const int* __restrict__ slot_of,
                                             long long* __restrict__ F, int* __restrict__ flags) {
     int r = blockIdx.y, lane = threadIdx.x;
     double sp = rec[r].sp;
     long long* Fr = F + (size_t)r * 3 * Np;
+    const int* S = slot_of + (size_t)r * Np;
     int bad = 0;
     if ((int)blockIdx.x < maxc) {
         int c = blockIdx.x;
@@ -765,13 +966,13 @@ __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
         int b = orig_s[(size_t)r * Np + s];
         if (cls[b] != 0) return;
         double4 d = dB[((size_t)r * maxc + c) * 64 + lane];
-        Fr[b] += to_fixed(sp * d.x, &bad);
-        Fr[Np + b] += to_fixed(sp * d.y, &bad);
-        Fr[2 * Np + b] += to_fixed(sp * d.z, &bad);
+        Fr[s] += to_fixed(sp * d.x, &bad);
+        Fr[Np + s] += to_fixed(sp * d.y, &bad);
+        Fr[2 * Np + s] += to_fixed(sp * d.z, &bad);
     } else {
         int li = (blockIdx.x - maxc) * 64 + lane;
         if (li >= L) return;
-        int a = lig[li];
+        int a = S[lig[li]];
         double4 d = dlig[(size_t)r * L + li];
         Fr[a] += to_fixed(sp * d.x, &bad);
         Fr[Np + a] += to_fixed(sp * d.y, &bad);
@@ -785,7 +986,8 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
                                                   const long long* __restrict__ F,
                                                   double* __restrict__ x64, double* __restrict__ v64,
                                                   const DevRecord* __restrict__ rec,
-                                                  const DevState* __restrict__ state) {
+                                                  const DevState* __restrict__ state,
+                                                  const int* __restrict__ slot_of) {
     int r = blockIdx.y;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -800,9 +1002,10 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
     double* X = x64 + (size_t)r * 3 * Np;
     double* V = v64 + (size_t)r * 3 * Np;
     double inv = 1.0 / dt;
+    int slot = slot_of[(size_t)r * Np + i];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double f = from_fixed(Fr[d * Np + i]);
+        double f = from_fixed(Fr[d * Np + slot]);
         double vel = vscale * V[d * Np + i] + fscale * im * f + noise * sim * gz[d];
         double delta = dt * vel;
         X[d * Np + i] += delta;

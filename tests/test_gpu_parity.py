@@ -26,8 +26,9 @@ def close(a, b, tol=E_TOL, floor=1.0):
     return abs(a - b) <= tol * max(floor, abs(b))
 
 
-def gpu_components(s, params=None, displacement=None):
+def gpu_components(s, params=None, displacement=None, newton=1):
     ctx = _native.NativeContext(s, 1, displacement=displacement)
+    ctx.set_tuning(pair_newton3=newton)
     ctx.set_params(0, params or systems.params_config1())
     ctx.set_positions(0, s.positions)
     ctx.set_velocities(0, s.velocities)
@@ -37,9 +38,9 @@ def gpu_components(s, params=None, displacement=None):
     return ctx, e, F
 
 
-def check_components(s, params=None):
-    print(f"case {s.name} n={s.n}", flush=True)
-    ctx, e, F = gpu_components(s, params)
+def check_components(s, params=None, newton=1):
+    print(f"case {s.name} n={s.n} newton={newton}", flush=True)
+    ctx, e, F = gpu_components(s, params, newton=newton)
     topo = O.Topology(s)
     E1, E2, Eb, F1, F2, Fb = O.reference_eval(topo, s.positions)
     U0, u, F0d, dF, _, _ = O.sdm_direct(topo, s.positions)
@@ -67,23 +68,27 @@ def check_components(s, params=None):
     return ctx
 
 
+@pytest.mark.parametrize("newton", [1, 0])
 @pytest.mark.parametrize("frame", [0, 1, 2])
-def test_small_components(hip_device, frame):
-    check_components(systems.make_small(frame=frame))
+def test_small_components(hip_device, frame, newton):
+    check_components(systems.make_small(frame=frame), newton=newton)
 
 
+@pytest.mark.parametrize("newton", [1, 0])
 @pytest.mark.parametrize("frame", [0, 1])
-def test_config1_components(hip_device, frame):
-    check_components(systems.make_config1(frame=frame))
+def test_config1_components(hip_device, frame, newton):
+    check_components(systems.make_config1(frame=frame), newton=newton)
 
 
-def test_config2_components(hip_device):
-    check_components(systems.make_config2(frame=0), systems.params_config2())
+@pytest.mark.parametrize("newton", [1, 0])
+def test_config2_components(hip_device, newton):
+    check_components(systems.make_config2(frame=0), systems.params_config2(), newton=newton)
 
 
-def test_config5_components(hip_device):
+@pytest.mark.parametrize("newton", [1, 0])
+def test_config5_components(hip_device, newton):
     s = systems.make_config5(frame=0)
-    check_components(s, systems.params_config5()[3])
+    check_components(s, systems.params_config5()[3], newton=newton)
 
 
 @pytest.mark.parametrize("bias", [0, 1, 2])
@@ -161,9 +166,11 @@ def test_nonequilibrium_schedule(hip_device):
     assert close(e["pot_energy"], rep.s.pot_energy)
 
 
-def test_determinism_bitwise(hip_device):
+@pytest.mark.parametrize("newton", [1, 0])
+def test_determinism_bitwise(hip_device, newton):
     s = systems.make_config1(frame=3)
     ctx = _native.NativeContext(s, 1)
+    ctx.set_tuning(pair_newton3=newton)
     ctx.set_params(0, systems.params_config1())
     ctx.set_positions(0, s.positions)
     ctx.compute()
@@ -202,9 +209,11 @@ def test_batched_equals_single(hip_device):
         assert np.array_equal(multi.forces(k), single.forces(0))
 
 
-def test_pair_counts(hip_device):
+@pytest.mark.parametrize("newton", [1, 0])
+def test_pair_counts(hip_device, newton):
     s = systems.make_config1(frame=0)
     ctx = _native.NativeContext(s, 1)
+    ctx.set_tuning(pair_newton3=newton)
     ctx.set_params(0, systems.params_config1())
     ctx.set_positions(0, s.positions)
     p0, pd = ctx.count_pairs()
