@@ -216,7 +216,8 @@ int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);
    fitting): for every frame f of d_frames ([nframes][n_atoms][3] device doubles) and
    every selected atom s, writes du/dq, du/dsigma, du/deps to d_grad
    ([nframes][nsel][3] device doubles) and u to d_u ([nframes] device doubles).
-   Pair terms are recomputed, never stored.  Only pairs with d_i != d_j contribute. */
+   Pair terms are recomputed, never stored.  Only pairs with d_i != d_j contribute.
+   Either output may be NULL (u alone is the forward pass, the gradient the backward). */
 int sdm_du_dparams_frames(sdm_context* ctx, const double* d_frames, int32_t nframes,
                           const int32_t* sel, int32_t nsel, double* d_grad, double* d_u);
 

@@ -83,6 +83,8 @@ struct sdm_context {
     double* dFd = nullptr;
     unsigned long long* counters = nullptr;
     double* ke_out = nullptr;
+    int* sel_dev = nullptr;
+    size_t sel_cap = 0;
     sdm_tuning tune{1, 0, 1, 1};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -959,8 +961,29 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
 
 int sdm_du_dparams_frames(sdm_context* c, const double* d_frames, int32_t nframes,
                           const int32_t* sel, int32_t nsel, double* d_grad, double* d_u) {
-    (void)c; (void)d_frames; (void)nframes; (void)sel; (void)nsel; (void)d_grad; (void)d_u;
-    return fail(SDM_ERR_UNSUPPORTED, "adjoint not built yet");
+    if (!c || !d_frames || nframes <= 0) return fail(SDM_ERR_INVALID, "bad argument");
+    if (d_grad && (!sel || nsel <= 0)) return fail(SDM_ERR_INVALID, "gradient needs selected atoms");
+    if (d_u && c->L > 0)
+        k_u_frames<<<nframes, 256, 0, c->stream>>>(c->N, c->L, c->lig, c->cls, c->cdispl, c->q64,
+                                                   c->sig64, c->eps64, c->ex_start, c->ex_list,
+                                                   d_frames, d_u, c->g);
+    else if (d_u)
+        k_fill_d<<<(nframes + 255) / 256, 256, 0, c->stream>>>(d_u, (size_t)nframes);
+    if (d_grad) {
+        for (int k = 0; k < nsel; k++)
+            if (sel[k] < 0 || sel[k] >= c->N) return fail(SDM_ERR_INVALID, "selected atom out of range");
+        if ((size_t)nsel > c->sel_cap) {
+            CHK(dalloc(c, &c->sel_dev, nsel));
+            c->sel_cap = nsel;
+        }
+        HIPCHK(hipMemcpyAsync(c->sel_dev, sel, sizeof(int) * nsel, hipMemcpyHostToDevice, c->stream));
+        k_du_dparams<<<dim3(nsel, nframes), 64, 0, c->stream>>>(
+            c->N, c->sel_dev, nsel, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
+            c->ex_list, d_frames, d_grad, c->g);
+    }
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
 }
 
 void registerPlatforms(void) {}

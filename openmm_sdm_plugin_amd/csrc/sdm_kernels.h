@@ -1034,6 +1034,105 @@ __global__ __launch_bounds__(256) void k_kinetic(int N, int Np, const double* __
 }
 
 // ---------------------------------------------------------------------------
+// adjoint (parameter gradients of u over many frames).  Frames are [F][N][3] doubles in
+// atom order; pair terms are recomputed from the positions, never stored.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_frame_atom(const double* __restrict__ X, int a, double& x,
+                                                double& y, double& z) {
+    x = X[3 * a];
+    y = X[3 * a + 1];
+    z = X[3 * a + 2];
+}
+
+// u of every frame: one workgroup per frame, pairs (displaced atom i, atom j) with
+// d_i != d_j, displaced-displaced pairs once
+__global__ __launch_bounds__(256) void k_u_frames(
+    int N, int L, const int* __restrict__ lig, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ q64,
+    const double* __restrict__ sig64, const double* __restrict__ eps64,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const double* __restrict__ frames, double* __restrict__ u_out, Geom g) {
+    __shared__ double red[256];
+    int f = blockIdx.x, t = threadIdx.x;
+    const double* X = frames + (size_t)f * 3 * N;
+    double acc = 0.0;
+    for (long long idx = t; idx < (long long)L * N; idx += 256) {
+        int i = lig[idx / N];
+        int j = (int)(idx % N);
+        int ci = cls[i], cj = cls[j];
+        if (ci == cj || (cj != 0 && j < i)) continue;
+        if (excluded(ex_start, ex_list, i, j)) continue;
+        Atom64 I, J;
+        load_frame_atom(X, i, I.x, I.y, I.z);
+        load_frame_atom(X, j, J.x, J.y, J.z);
+        I.dx = cdispl[3 * ci]; I.dy = cdispl[3 * ci + 1]; I.dz = cdispl[3 * ci + 2];
+        J.dx = cdispl[3 * cj]; J.dy = cdispl[3 * cj + 1]; J.dz = cdispl[3 * cj + 2];
+        I.qk = q64[i]; I.sig = sig64[i]; I.eps = eps64[i];
+        J.qk = q64[j]; J.sig = sig64[j]; J.eps = eps64[j];
+        double fx = 0, fy = 0, fz = 0;
+        int iny;
+        acc += delta_pair(I, J, g, fx, fy, fz, &iny);
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) u_out[f] = red[0];
+}
+
+// du/d(q_s, sigma_s, eps_s) for selected atom s in frame f: one wave per (s, f)
+__global__ __launch_bounds__(64) void k_du_dparams(
+    int N, const int* __restrict__ sel, int nsel, const int* __restrict__ cls,
+    const double* __restrict__ cdispl, const double* __restrict__ q64,
+    const double* __restrict__ sig64, const double* __restrict__ eps64,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const double* __restrict__ frames, double* __restrict__ grad, Geom g) {
+    int k = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+    int s = sel[k];
+    const double* X = frames + (size_t)f * 3 * N;
+    int cs = cls[s];
+    double xs, ys, zs;
+    load_frame_atom(X, s, xs, ys, zs);
+    const double ske = sqrt(SDM_ONE_4PI_EPS0);
+    double dq = 0.0, dsig = 0.0, deps = 0.0;
+    for (int j = lane; j < N; j += 64) {
+        int cj = cls[j];
+        if (cj == cs || excluded(ex_start, ex_list, s, j)) continue;
+        double xj, yj, zj;
+        load_frame_atom(X, j, xj, yj, zj);
+        double sig = 0.5 * (sig64[s] + sig64[j]);
+        double eps = sqrt(eps64[s] * eps64[j]);
+        for (int geo = 0; geo < 2; geo++) {  // geo 0: at x (sign -), geo 1: at x+d (sign +)
+            double dx = min_image((xs + (geo ? cdispl[3 * cs] : 0.0)) - (xj + (geo ? cdispl[3 * cj] : 0.0)), g.Lx, g.iLx);
+            double dy = min_image((ys + (geo ? cdispl[3 * cs + 1] : 0.0)) - (yj + (geo ? cdispl[3 * cj + 1] : 0.0)), g.Ly, g.iLy);
+            double dz = min_image((zs + (geo ? cdispl[3 * cs + 2] : 0.0)) - (zj + (geo ? cdispl[3 * cj + 2] : 0.0)), g.Lz, g.iLz);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < g.rc2)) continue;
+            double sgn = geo ? 1.0 : -1.0;
+            double ir = 1.0 / sqrt(r2);
+            double s2 = sig * sig / r2;
+            double s6 = s2 * s2 * s2;
+            double s12 = s6 * s6;
+            // q64 holds q*sqrt(k_e): d(k_e q_s q_j f)/dq_s = sqrt(k_e) * q64[j] * f
+            dq += sgn * ske * q64[j] * (ir + g.krf * r2 - g.crf);
+            dsig += sgn * 0.5 * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
+            if (eps > 0.0) deps += sgn * 4.0 * (s12 - s6) * eps64[j] / (2.0 * eps);
+        }
+    }
+    dq = wave_sum(dq);
+    dsig = wave_sum(dsig);
+    deps = wave_sum(deps);
+    if (lane == 0) {
+        double* o = grad + ((size_t)f * nsel + k) * 3;
+        o[0] = dq;
+        o[1] = dsig;
+        o[2] = deps;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // diagnostics: dense component forces (FP64) for sdm_compute_components.
 // ---------------------------------------------------------------------------
 __global__ void k_fill_ll(long long* __restrict__ p, size_t n) {

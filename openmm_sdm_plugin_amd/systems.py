@@ -425,6 +425,14 @@ def make_config4(n_frames: int = 64) -> List[SDMSystem]:
     return [make_config1(frame=k) for k in range(n_frames)]
 
 
+def params_config4() -> SDMParams:
+    """Config 1's soft core with an ILogistic bias whose gradients in alpha, u0 and w0
+    are all nonzero (lambda1 != lambda2, alpha > 0)."""
+    return SDMParams(bias_method=BIAS_ILOGISTIC, softcore_method=SOFTCORE_RATIONAL,
+                     lambda_=0.4, lambda1=0.2, lambda2=0.6, alpha=0.1, u0=110.0, w0coeff=0.0,
+                     umax=200.0, acore=0.0625, ubcore=100.0)
+
+
 def make_config5(frame: int = 0, topo_seed: int = 5) -> SDMSystem:
     """99,920 atoms: 33,000 waters + 800-atom host shell + guest A (60, d=(+3,0,0)) inside
     the host and guest B (60, d=(-3,0,0)) 3 nm away in solvent; 10 nm box, rc 1.2 nm."""
