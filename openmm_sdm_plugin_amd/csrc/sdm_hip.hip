@@ -47,6 +47,10 @@ int fail(int code, const std::string& msg) {
 struct sdm_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // side streams of the step's fork/join: bonded terms and the displaced-state chain
+    // run concurrently with the pair kernel
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
     int NC = 0, ncx = 0, ncy = 0, ncz = 0, NC_alloc = 0;
     int maxj = 0, maxc = 0, nbond = 0, nang = 0, nbw = 1;
@@ -250,34 +254,47 @@ void enqueue_pair(sdm_context* c, bool count) {
 #undef SDM_LAUNCH_PAIR
 }
 
-// evaluation of one step (everything before the Langevin update)
-void enqueue_eval(sdm_context* c, bool advance, bool components) {
-    hipStream_t s = c->stream;
+// evaluation of one step (everything before the Langevin update).  After the sort the
+// work forks: the pair kernel on the main stream, the bonded terms and the displaced-state
+// chain on two side streams; they join before the scalar stage.  pe0/pe1 (optional) are
+// recorded around the pair kernel.
+void enqueue_eval(sdm_context* c, bool advance, bool components, hipEvent_t pe0 = nullptr,
+                  hipEvent_t pe1 = nullptr) {
+    hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
     enqueue_sort(c);
-    enqueue_pair(c, false);
+    (void)hipEventRecord(c->ev_fork, s);
+    (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
+    (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
     if (c->nbond + c->nang > 0)
-        k_bonded<<<dim3(c->nbw, R), 256, 0, s>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
-                                                 c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
-                                                 c->x64, components ? nullptr : c->slot_of,
-                                                 components ? c->Fb_sep : c->F, c->e_bond,
-                                                 c->nbw, c->flags);
+        k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
+                                                  c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
+                                                  c->x64, components ? nullptr : c->slot_of,
+                                                  components ? c->Fb_sep : c->F, c->e_bond,
+                                                  c->nbw, c->flags);
     if (c->L > 0) {
-        k_ligcand<<<R, 64, 0, s>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
-                                   c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
-                                   c->g, c->flags);
-        k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, s>>>(
+        k_ligcand<<<R, 64, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
+                                    c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
+                                    c->g, c->flags);
+        k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, s2>>>(
             N, Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
             nullptr);
-        k_deltaA_red<<<dim3(c->LB, R), 64, 0, s>>>(
+        k_deltaA_red<<<dim3(c->LB, R), 64, 0, s2>>>(
             Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
             c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g, nullptr);
-        k_deltaB<<<dim3(c->maxc, R), 64, 0, s>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
-                                                 c->sig64, c->eps64, c->ex_start, c->ex_list,
-                                                 c->x64, c->orig_s, c->cand, c->ncand, c->maxc,
-                                                 c->dB, c->g);
+        k_deltaB<<<dim3(c->maxc, R), 64, 0, s2>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
+                                                  c->sig64, c->eps64, c->ex_start, c->ex_list,
+                                                  c->x64, c->orig_s, c->cand, c->ncand, c->maxc,
+                                                  c->dB, c->g);
     }
+    if (pe0) (void)hipEventRecord(pe0, s);
+    enqueue_pair(c, false);
+    if (pe1) (void)hipEventRecord(pe1, s);
+    (void)hipEventRecord(c->ev_join[0], s1);
+    (void)hipEventRecord(c->ev_join[1], s2);
+    (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
+    (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, c->hist_every, c->hist_cap,
                                c->uhist);
@@ -287,8 +304,8 @@ void enqueue_eval(sdm_context* c, bool advance, bool components) {
                                                        c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
-void enqueue_step(sdm_context* c) {
-    enqueue_eval(c, true, false);
+void enqueue_step(sdm_context* c, hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
+    enqueue_eval(c, true, false, pe0, pe1);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
                                                            c->v64, c->rec, c->state, c->slot_of);
 }
@@ -457,8 +474,13 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         return code;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SDM_ERR_HIP, "no HIP device"));
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(SDM_ERR_HIP, "hipStreamCreate failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(SDM_ERR_HIP, "hipStreamCreate/hipEventCreate failed"));
     c->N = N;
     c->Np = (N + 63) / 64 * 64;
     c->NB = c->Np / 64;
@@ -560,6 +582,10 @@ int sdm_destroy(sdm_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (void* p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : {c->ev_fork, c->ev_join[0], c->ev_join[1]})
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t st : {c->side[0], c->side[1]})
+        if (st) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SDM_OK;
@@ -840,46 +866,10 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e1));
     double total = 0.0;
     for (int s = 0; s < nsteps; s++) {
-        // the step of enqueue_step, with events around the pair kernel
-        enqueue_sort(c);
-        HIPCHK(hipEventRecord(e0, c->stream));
-        enqueue_pair(c, false);
-        HIPCHK(hipEventRecord(e1, c->stream));
-        hipStream_t st = c->stream;
-        int N = c->N, Np = c->Np, R = c->R;
-        if (c->nbond + c->nang > 0)
-            k_bonded<<<dim3(c->nbw, R), 256, 0, st>>>(Np, c->nbond, c->nang, c->bond_atoms,
-                                                      c->bond_r0, c->bond_k, c->ang_atoms,
-                                                      c->ang_t0, c->ang_k, c->x64, c->slot_of,
-                                                      c->F, c->e_bond, c->nbw, c->flags);
-        if (c->L > 0) {
-            k_ligcand<<<R, 64, 0, st>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
-                                        c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
-                                        c->list_r2, c->g, c->flags);
-            k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, st>>>(
-                N, Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
-                c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA,
-                c->g, nullptr);
-            k_deltaA_red<<<dim3(c->LB, R), 64, 0, st>>>(
-                Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
-                c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part,
-                c->g, nullptr);
-            k_deltaB<<<dim3(c->maxc, R), 64, 0, st>>>(N, Np, c->L, c->lig, c->cls, c->cdispl,
-                                                      c->q64, c->sig64, c->eps64, c->ex_start,
-                                                      c->ex_list, c->x64, c->orig_s, c->cand,
-                                                      c->ncand, c->maxc, c->dB, c->g);
-        }
-        k_scalar<<<R, 256, 0, st>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part,
-                                    c->state, c->rec, c->flags, 1, c->hist_every, c->hist_cap,
-                                    c->uhist);
-        if (c->L > 0)
-            k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, st>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
-                                                            c->orig_s, c->cand, c->ncand, c->dB,
-                                                            c->dlig, c->rec, c->slot_of, c->F, c->flags);
-        k_langevin<<<g2(N, 256, R), 256, 0, st>>>(N, Np, c->invm, c->F, c->x64, c->v64, c->rec,
-                                                  c->state, c->slot_of);
+        enqueue_step(c, e0, e1);
         CHK(check_launch());
         HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         total += ms;

@@ -227,20 +227,20 @@ __global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
 // the hot kernel: U0 and F0 over all pairs at x.  One wave per (replica, i-block):
 // lane = i atom (registers), j atoms are wave-uniform (scalar loads) taken from each
 // tile's j mask.
-//   NEWTON = false: full tile list, every ordered pair once, i force stored (no atomics),
-//                   energy halved.
+//   NEWTON = false: full tile list, every ordered pair once, no j forces, energy halved.
 //   NEWTON = true:  half tile list (j-block >= i-block, j > i inside the diagonal tile),
 //                   every pair once; the j forces of 8 j atoms at a time are summed over
 //                   the wave by a reduce-scatter and added with int64 atomics.
 // When a tile's two boxes fit in half a box length the periodic image is one shift for
 // the whole tile (applied to the i coordinates); otherwise each pair takes the minimum
-// image.  Pair energies are accumulated in FP64 per pair and the per-tile FP32 force sums
-// in FP64 in tile order; fixed-point atomics commute, so results are bit-reproducible.
+// image.  Pair energies are accumulated in FP64 per pair and the per-tile FP32 force and
+// reaction-field sums in FP64 in tile order; fixed-point atomics commute, so results are
+// bit-reproducible.
 // ---------------------------------------------------------------------------
 struct PairAcc {
     float fx, fy, fz;  // per-tile force on the lane's i atom
     float sqr2, sq;    // per-tile sums of qq r^2 and qq (reaction-field energy terms)
-    double e;          // LJ + qq/r energy
+    double e;          // LJ + qq/r energy, FP64 per pair (net energies cancel strongly)
     int cnt;
 };
 
@@ -313,32 +313,40 @@ __device__ __forceinline__ void tile_full(const float4* __restrict__ P, const fl
     }
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
 // Sum v[24] (8 j atoms x 3 components, index 3k+c) over the 64 lanes so that lanes
-// 8k..8k+7 end up holding the three components of j atom k: three halving exchange
-// stages (lane bits 5, 4, 3) then a full sum over lane bits 2..0.
+// 8k..8k+7 end up holding the three components of j atom k.  Three halving stages split
+// the values between lane halves (bit 5: v_permlane32_swap, bit 4: v_permlane16_swap,
+// bit 3: DPP row_ror:8), then a DPP sum over the 8 lanes of each group.  The swaps
+// exchange the upper half of their first operand with the lower half of their second,
+// so r0 + r1 is the pairwise sum of the value each half keeps, with no selects.
 __device__ __forceinline__ void reduce_scatter24(const float (&v)[24], int lane, float (&out)[3]) {
     float w[12], x[6];
-    bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
 #pragma unroll
     for (int t = 0; t < 12; t++) {
-        float keep = b5 ? v[t + 12] : v[t];
-        float send = b5 ? v[t] : v[t + 12];
-        w[t] = keep + __shfl_xor(send, 32, 64);
+        auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t]), __float_as_uint(v[t + 12]),
+                                                   false, false);
+        w[t] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
 #pragma unroll
     for (int t = 0; t < 6; t++) {
-        float keep = b4 ? w[t + 6] : w[t];
-        float send = b4 ? w[t] : w[t + 6];
-        x[t] = keep + __shfl_xor(send, 16, 64);
+        auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[t]), __float_as_uint(w[t + 6]),
+                                                   false, false);
+        x[t] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
+    bool b3 = (lane & 8) != 0;
 #pragma unroll
     for (int t = 0; t < 3; t++) {
         float keep = b3 ? x[t + 3] : x[t];
         float send = b3 ? x[t] : x[t + 3];
-        float y = keep + __shfl_xor(send, 8, 64);
-        y += __shfl_xor(y, 4, 64);
-        y += __shfl_xor(y, 2, 64);
-        y += __shfl_xor(y, 1, 64);
+        float y = keep + dpp_f<0x128>(send);  // row_ror:8 = lane ^ 8 within 16
+        y += dpp_f<0xB1>(y);                  // quad_perm [1,0,3,2] = lane ^ 1
+        y += dpp_f<0x4E>(y);                  // quad_perm [2,3,0,1] = lane ^ 2
+        y += dpp_f<0x141>(y);                 // row_half_mirror: lane i <- 7 - i within 8
         out[t] = y;
     }
 }
@@ -484,17 +492,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
         if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(c + 0.5));
         return;
     }
-    if (vi) {
+    if (vi) {  // atomic in both modes: the bonded kernel may run concurrently
         long long fx = to_fixed(ax, &bad), fy = to_fixed(ay, &bad), fz = to_fixed(az, &bad);
-        if (NEWTON) {
-            atomicAdd(&Fr[islot], (unsigned long long)fx);
-            atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
-            atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
-        } else {
-            Fr[islot] = (unsigned long long)fx;
-            Fr[Np + islot] = (unsigned long long)fy;
-            Fr[2 * Np + islot] = (unsigned long long)fz;
-        }
+        atomicAdd(&Fr[islot], (unsigned long long)fx);
+        atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
+        atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
     }
     double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
