@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile", type=int, default=0,
+                    help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
     return ap.parse_args()
 
 
@@ -99,6 +101,14 @@ def main():
         ctx.set_positions(r, frames[r].positions)
         ctx.set_velocities(r, frames[r].velocities)
     ctx.set_u_history(100, 4096)
+    if a.profile:
+        # one launch per kernel (no graph) so a profiler attributes every dispatch
+        ctx.set_tuning(use_graph=0)
+        ctx.step(a.profile)
+        e = ctx.energies()
+        print(json.dumps({"profile_steps": a.profile, "step": e[0]["step"],
+                          "flags": [x["flags"] for x in e], "tuning": ctx.get_tuning()}))
+        return
     tuning = ctx.autotune(20) if not a.no_autotune else ctx.get_tuning()
 
     # ---- batched ladder: warm-up then exactly K timed steps ----
