@@ -126,8 +126,12 @@ typedef struct {
     int32_t pair_waves_per_group; /* 1, 2 or 4 wave64s per workgroup in the pair kernel */
     int32_t cell_cells_per_dim;   /* sort-grid resolution; 0 = automatic */
     int32_t use_graph;            /* replay steps from a captured hipGraph */
-    int32_t pair_newton3;         /* 1: each pair once (half tile list, j-force scatter);
-                                     0: each ordered pair (full tile list, no atomics) */
+    int32_t pair_newton3;         /* block kernel only.  1: each pair once (half tile list,
+                                     j-force scatter); 0: each ordered pair (no atomics) */
+    int32_t pair_group;           /* 0: block kernel (one 64-atom i-block per wave, shared
+                                     j stream); 16 or 8: grouped kernel (the wave's i-block
+                                     split into groups of 16 or 8 atoms, each with its own j
+                                     stream, each pair once) */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -55,6 +55,20 @@ __device__ __forceinline__ long long to_fixed(double f, int* bad) {
     return __double2ll_rn(f * SDM_FIX_SCALE);
 }
 
+// The same fixed point from an FP32 value: floor(f) is the high word and the fraction
+// times 2^32 (exact in FP32, truncated below 2^-32) the low word.  Out of range or NaN
+// sets *bad and clamps like to_fixed.
+__device__ __forceinline__ unsigned long long fixed_from_float(float f, int* bad) {
+    if (!(fabsf(f) < 2147483520.0f)) {  // largest FP32 below 2^31
+        *bad = 1;
+        f = (f != f) ? 0.0f : copysignf(2147483520.0f, f);
+    }
+    float fl = floorf(f);
+    unsigned hi = (unsigned)(int)fl;
+    unsigned lo = (unsigned)((f - fl) * 4294967296.0f);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 __device__ __forceinline__ double from_fixed(long long v) { return (double)v * (1.0 / SDM_FIX_SCALE); }
 
 __device__ __forceinline__ double wave_sum(double v) {

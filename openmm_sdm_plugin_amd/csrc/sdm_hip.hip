@@ -53,7 +53,7 @@ struct sdm_context {
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
     int NC = 0, ncx = 0, ncy = 0, ncz = 0, NC_alloc = 0;
-    int maxj = 0, maxc = 0, nbond = 0, nang = 0, nbw = 1;
+    int maxj = 0, maxc = 0, nbond = 0, nang = 0, nbw = 1, max_ex = 0;
     Geom g{};
     float list_r2 = 0.f;
     // topology (shared by all replicas)
@@ -69,6 +69,7 @@ struct sdm_context {
     double *x64 = nullptr, *v64 = nullptr;
     float4* posq_s = nullptr;
     float2* par_s = nullptr;
+    float4* arec = nullptr;  // [R][Np][2] slot records of the grouped pair kernel
     int *orig_s = nullptr, *slot_of = nullptr, *key = nullptr;
     int *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
     float4 *bb_lo = nullptr, *bb_hi = nullptr;
@@ -89,7 +90,7 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
-    sdm_tuning tune{1, 0, 1, 1};
+    sdm_tuning tune{2, 0, 1, 1, 16};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool components_valid = false;
@@ -218,17 +219,71 @@ void enqueue_sort(sdm_context* c) {
     k_scatter<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->NC, c->key, c->cstart, c->cfill, c->orig_s);
     k_cellsort<<<g2(c->NC, 128, R), 128, 0, s>>>(N, Np, c->NC, c->cstart, c->orig_s, c->slot_of,
                                                 c->x64, c->qs, c->sh, c->se, c->posq_s,
-                                                c->par_s, c->F, c->g);
+                                                c->par_s, c->arec, c->F, c->g);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
-    k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi,
-                                          c->tiles, c->tmask, c->ntiles, c->list_r2, c->g,
-                                          c->tune.pair_newton3, c->flags);
+    if (c->tune.pair_group == 0)  // the grouped kernel finds its j-blocks itself
+        k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo,
+                                              c->bb_hi, c->tiles, c->tmask, c->ntiles,
+                                              c->list_r2, c->g, c->tune.pair_newton3, c->flags);
+}
+
+// ring capacity (entries per group) of the grouped pair kernel
+template <int G>
+constexpr int grp_cap() {
+    return G == 16 ? 256 : 128;
+}
+
+template <int G, int W>
+void launch_pair_grp(sdm_context* c, bool count, unsigned long long* pc) {
+    constexpr int CAP = grp_cap<G>();
+    int waves = c->R * c->NB;
+    dim3 gr((waves + W - 1) / W);
+    // per wave: rings, exclusion partners, group boxes (the staging buffers are static)
+    size_t lds = (size_t)W * ((64 / G) * (CAP + 8) + c->max_ex * 64) * sizeof(unsigned);
+#define SDM_GRP_ARGS                                                                            \
+    c->R, c->N, c->Np, c->NB, c->arec, c->orig_s, c->slot_of, c->bb_lo, c->bb_hi, c->ex_start, \
+        c->ex_list, c->max_ex, c->F, c->e_pair, c->flags, pc, c->list_r2, c->g
+    if (count) k_pair_grp<G, CAP, W, true><<<gr, 64 * W, lds, c->stream>>>(SDM_GRP_ARGS);
+    else k_pair_grp<G, CAP, W, false><<<gr, 64 * W, lds, c->stream>>>(SDM_GRP_ARGS);
+#undef SDM_GRP_ARGS
+}
+
+template <int G>
+void launch_pair_grp_w(sdm_context* c, bool count, unsigned long long* pc) {
+    switch (c->tune.pair_waves_per_group) {
+        case 2: launch_pair_grp<G, 2>(c, count, pc); break;
+        case 4: launch_pair_grp<G, 4>(c, count, pc); break;
+        default: launch_pair_grp<G, 1>(c, count, pc); break;
+    }
 }
 
 void enqueue_pair(sdm_context* c, bool count) {
     hipStream_t s = c->stream;
     int waves = c->R * c->NB;
     unsigned long long* pc = count ? c->counters : nullptr;
+    if (c->tune.pair_group == 16) return launch_pair_grp_w<16>(c, count, pc);
+    if (c->tune.pair_group == 8) return launch_pair_grp_w<8>(c, count, pc);
+    if (c->tune.pair_newton3) {
+        size_t lds = (size_t)c->tune.pair_waves_per_group * c->max_ex * 64 * sizeof(int);
+#define SDM_N3_ARGS                                                                             \
+    c->R, c->N, c->Np, c->NB, c->maxj, c->arec, c->orig_s, c->slot_of, c->tiles, c->tmask,      \
+        c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->max_ex, c->F, c->e_pair, \
+        c->flags, pc, c->g
+#define SDM_LAUNCH_N3(W)                                                                      \
+    do {                                                                                      \
+        dim3 gr((waves + W - 1) / W);                                                         \
+        if (count) k_pair_n3<W, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                  \
+        else k_pair_n3<W, false><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                       \
+    } while (0)
+        switch (c->tune.pair_waves_per_group) {
+            case 2: SDM_LAUNCH_N3(2); break;
+            case 4: SDM_LAUNCH_N3(4); break;
+            default: SDM_LAUNCH_N3(1); break;
+        }
+#undef SDM_LAUNCH_N3
+#undef SDM_N3_ARGS
+        return;
+    }
 #define SDM_PAIR_ARGS                                                                       \
     c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, c->tiles, \
         c->tmask, c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->F, c->e_pair,   \
@@ -406,6 +461,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     *out = nullptr;
     int N = sys->n_atoms;
     if (N <= 0) return fail(SDM_ERR_INVALID, "n_atoms must be positive");
+    // the grouped pair kernel addresses 32-byte slot records with 32-bit byte offsets
+    if (N >= (1 << 26)) return fail(SDM_ERR_UNSUPPORTED, "n_atoms must be below 2^26");
     if (n_replicas <= 0) return fail(SDM_ERR_INVALID, "n_replicas must be positive");
     if (!sys->charge || !sys->sigma || !sys->epsilon || !sys->mass || !sys->displacement)
         return fail(SDM_ERR_INVALID, "per-atom arrays must not be null");
@@ -426,9 +483,11 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         ecount[a + 1]++;
         ecount[b + 1]++;
     }
+    int max_ex = 0;
     for (int i = 0; i < N; i++) {
         if (ecount[i + 1] > SDM_MAX_EXCL)
             return fail(SDM_ERR_UNSUPPORTED, "more than 16 exclusions on one atom");
+        max_ex = std::max(max_ex, ecount[i + 1]);
         ecount[i + 1] += ecount[i];
     }
     std::vector<int> elist(std::max(1, ecount[N])), efill(N, 0);
@@ -482,6 +541,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess)
         return bail(fail(SDM_ERR_HIP, "hipStreamCreate/hipEventCreate failed"));
     c->N = N;
+    c->max_ex = max_ex;
     c->Np = (N + 63) / 64 * 64;
     c->NB = c->Np / 64;
     c->R = n_replicas;
@@ -524,7 +584,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(bond_atoms, 2 * std::max(1, c->nbond)); A(bond_r0, std::max(1, c->nbond));
     A(bond_k, std::max(1, c->nbond)); A(ang_atoms, 3 * std::max(1, c->nang));
     A(ang_t0, std::max(1, c->nang)); A(ang_k, std::max(1, c->nang));
-    A(x64, 3 * RNp); A(v64, 3 * RNp); A(posq_s, RNp); A(par_s, RNp); A(orig_s, RNp);
+    A(x64, 3 * RNp); A(v64, 3 * RNp); A(posq_s, RNp); A(par_s, RNp); A(arec, 2 * RNp);
+    A(orig_s, RNp);
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
@@ -853,7 +914,7 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
     std::vector<unsigned long long> h(n);
     CHK(download(c, h.data(), c->counters, n));
     for (int r = 0; r < c->R; r++) {
-        p0[r] = (int64_t)(c->tune.pair_newton3 ? h[r] : h[r] / 2);
+        p0[r] = (int64_t)((c->tune.pair_group || c->tune.pair_newton3) ? h[r] : h[r] / 2);
         pdelta[r] = (int64_t)h[c->R + r];
     }
     return SDM_OK;
@@ -893,6 +954,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_waves_per_group must be 1, 2 or 4");
     if (t->cell_cells_per_dim < 0 || t->cell_cells_per_dim > 1024)
         return fail(SDM_ERR_INVALID, "cell_cells_per_dim out of range");
+    if (t->pair_group != 0 && t->pair_group != 8 && t->pair_group != 16)
+        return fail(SDM_ERR_INVALID, "pair_group must be 0, 8 or 16");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -918,12 +981,15 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // pair kernels: block waves without / with Newton-3, grouped waves of 16 and of 8
+    const int kernels[][2] = {{0, 0}, {0, 1}, {16, 1}, {8, 1}};
     for (double side : cell_sides)
         for (int w : waves)
-        for (int newton = 0; newton < 2; newton++) {
+        for (const auto& kn : kernels) {
             sdm_tuning t = c->tune;
             t.pair_waves_per_group = w;
-            t.pair_newton3 = newton;
+            t.pair_group = kn[0];
+            t.pair_newton3 = kn[1];
             t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
             CHK(sdm_set_tuning(c, &t));
             CHK(sdm_step(c, 2));

@@ -18,6 +18,8 @@
 // U(x+d) - U(x) and F(x+d) - F(x) are sums over the few pairs with d_i != d_j only.
 // Every grid has the replica index in its y (or z) dimension.
 #pragma once
+#include <type_traits>
+
 #include "sdm_device.h"
 
 namespace sdm {
@@ -102,7 +104,7 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
                            const double* __restrict__ x64, const float* __restrict__ qs,
                            const float* __restrict__ sh, const float* __restrict__ se,
                            float4* __restrict__ posq_s, float2* __restrict__ par_s,
-                           long long* __restrict__ F, Geom g) {
+                           float4* __restrict__ arec, long long* __restrict__ F, Geom g) {
     int r = blockIdx.y;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= NC) return;
@@ -126,8 +128,12 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
         wx -= g.Lx * floor(wx * g.iLx);
         wy -= g.Ly * floor(wy * g.iLy);
         wz -= g.Lz * floor(wz * g.iLz);
-        posq_s[(size_t)r * Np + s] = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
+        float4 pq = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
+        posq_s[(size_t)r * Np + s] = pq;
         par_s[(size_t)r * Np + s] = make_float2(sh[i], se[i]);
+        // interleaved copy for the grouped pair kernel: one 32-byte record per slot
+        arec[2 * ((size_t)r * Np + s)] = pq;
+        arec[2 * ((size_t)r * Np + s) + 1] = make_float4(sh[i], se[i], 0.f, 0.f);
         long long* Fr = F + (size_t)r * 3 * Np;  // forces are accumulated in slot order
         Fr[s] = 0;
         Fr[Np + s] = 0;
@@ -167,10 +173,24 @@ __device__ __forceinline__ float bbox_gap2(float4 lo_i, float4 hi_i, float4 lo_j
     return gx * gx + gy * gy + gz * gz;
 }
 
+// min / max over the 8 lanes of each group of 8 (lane ^ 7, lane ^ 2, lane ^ 1 via DPP)
+__device__ __forceinline__ float min8(float v) {
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, true)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xf, 0xf, true)));
+    return fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xf, 0xf, true)));
+}
+__device__ __forceinline__ float max8(float v) {
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, true)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xf, 0xf, true)));
+    return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xf, 0xf, true)));
+}
+
 // tile list of one i-block: every j-block (j >= i for the half list) whose box is within
-// rc (+slack) of the i-block's box, ascending, each with a 64-bit mask of the j atoms that
-// lie within rc of that box (atoms outside it cannot have an in-cutoff partner in the
-// i-block).
+// rc (+slack) of the i-block's box, ascending, each with a 64-bit mask of the j atoms
+// that lie within rc of the box of at least one of the block's eight 8-slot groups
+// (atoms outside all of them cannot have an in-cutoff partner in the i-block; the
+// union of the small boxes is a much tighter region than the block's box).  Half list:
+// in the diagonal tile an atom only counts for groups that start below it.
 __global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
                                               const float4* __restrict__ posq_s,
                                               const float4* __restrict__ bb_lo,
@@ -179,13 +199,27 @@ __global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
                                               unsigned long long* __restrict__ tmask,
                                               int* __restrict__ ntiles, float list_r2, Geom g,
                                               int half, int* __restrict__ flags) {
+    __shared__ float4 sub[8][2];
     int r = blockIdx.y, ib = blockIdx.x, lane = threadIdx.x;
     const float4* lo = bb_lo + (size_t)r * NB;
     const float4* hi = bb_hi + (size_t)r * NB;
     const float4* P = posq_s + (size_t)r * Np;
     float4 li = lo[ib], hi_i = hi[ib];
-    float cx = 0.5f * (li.x + hi_i.x), cy = 0.5f * (li.y + hi_i.y), cz = 0.5f * (li.z + hi_i.z);
-    float hx = 0.5f * (hi_i.x - li.x), hy = 0.5f * (hi_i.y - li.y), hz = 0.5f * (hi_i.z - li.z);
+    // the eight group boxes (empty groups past N get an inverted box and never match)
+    {
+        int is = ib * 64 + lane;
+        bool v = is < N;
+        float4 p = P[min(is, N - 1)];
+        float lx = min8(v ? p.x : 3.0e38f), ly = min8(v ? p.y : 3.0e38f), lz = min8(v ? p.z : 3.0e38f);
+        float hx = max8(v ? p.x : -3.0e38f), hy = max8(v ? p.y : -3.0e38f), hz = max8(v ? p.z : -3.0e38f);
+        if ((lane & 7) == 0) {
+            bool any = lx <= hx;
+            sub[lane >> 3][0] = make_float4(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz), 0.f);
+            sub[lane >> 3][1] = any ? make_float4(0.5f * (hx - lx), 0.5f * (hy - ly), 0.5f * (hz - lz), 0.f)
+                                    : make_float4(-1.0e30f, -1.0e30f, -1.0e30f, 0.f);
+        }
+        __syncthreads();
+    }
     int* out = tiles + ((size_t)r * NB + ib) * maxj;
     unsigned long long* mout = tmask + ((size_t)r * NB + ib) * maxj;
     int count = 0;
@@ -200,13 +234,21 @@ __global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
             int jb2 = base + k;
             int s = jb2 * 64 + lane;
             float4 p = P[s];
-            float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-            dx -= g.bx * rintf(dx * g.ibx);
-            dy -= g.by * rintf(dy * g.iby);
-            dz -= g.bz * rintf(dz * g.ibz);
-            float gx = fmaxf(fabsf(dx) - hx, 0.f), gy = fmaxf(fabsf(dy) - hy, 0.f),
-                  gz = fmaxf(fabsf(dz) - hz, 0.f);
-            bool in = (s < N) && (gx * gx + gy * gy + gz * gz < list_r2);
+            bool in = false;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                float4 c = sub[q][0], h = sub[q][1];
+                float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+                dx -= g.bx * rintf(dx * g.ibx);
+                dy -= g.by * rintf(dy * g.iby);
+                dz -= g.bz * rintf(dz * g.ibz);
+                float gx = fmaxf(fabsf(dx) - h.x, 0.f), gy = fmaxf(fabsf(dy) - h.y, 0.f),
+                      gz = fmaxf(fabsf(dz) - h.z, 0.f);
+                bool inq = gx * gx + gy * gy + gz * gz < list_r2 && h.x >= 0.f;
+                if (half && jb2 == ib) inq = inq && s > ib * 64 + q * 8;
+                in = in || inq;
+            }
+            in = in && (s < N);
             unsigned long long am = __ballot(in);
             if (am != 0ull) {
                 if (lane == 0 && count < maxj) {
@@ -284,6 +326,72 @@ __device__ __forceinline__ float pair_one(float4 pi, float2 qi, float4 pj, float
     return fr;
 }
 
+// One pair, branch free: the result is computed for every lane and cleared with a bit
+// mask outside the cutoff (a never-taken branch per pair would cost more and would keep
+// the scheduler from overlapping consecutive pairs).  Masked lanes may hold inf or NaN
+// before the mask (r = 0 for an unused slot); the AND yields +0.
+__device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, float2 qj,
+                                             bool incl, bool minimg, const Geom& g, PairAcc& a,
+                                             float& dx, float& dy, float& dz) {
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+    if (minimg) {  // wave-uniform
+        dx -= g.bx * rintf(dx * g.ibx);
+        dy -= g.by * rintf(dy * g.iby);
+        dz -= g.bz * rintf(dz * g.ibz);
+    }
+    float r2 = dx * dx + dy * dy + dz * dz;
+    unsigned m = (incl && r2 < g.rc2f) ? 0xFFFFFFFFu : 0u;
+    float ir = __frsqrt_rn(r2);
+    float ir2 = ir * ir;
+    float sig = qi.x + qj.x;
+    float s2 = sig * sig * ir2;
+    float s6 = s2 * s2 * s2;
+    float es6 = (qi.y * qj.y) * s6;
+    float qq = pi.w * pj.w;
+    float fr = es6 * (12.0f * s6 - 6.0f) * ir2 + qq * (ir * ir2 - 2.0f * g.krff);
+    float ee = es6 * (s6 - 1.0f) + qq * ir;
+    fr = __uint_as_float(__float_as_uint(fr) & m);
+    ee = __uint_as_float(__float_as_uint(ee) & m);
+    float qqm = __uint_as_float(__float_as_uint(qq) & m);
+    a.fx += fr * dx;
+    a.fy += fr * dy;
+    a.fz += fr * dz;
+    a.e += (double)ee;
+    a.sqr2 += qqm * r2;
+    a.sq += qqm;
+    a.cnt += (int)(m & 1u);
+    // pin the sums here: otherwise the compiler sinks every pair's accumulation to the
+    // end of the chunk and keeps all eight pairs' operands live (VGPR pressure)
+    asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz), "+v"(a.e), "+v"(a.sqr2), "+v"(a.sq));
+    return fr;
+}
+
+// a + b taken from the DPP partner lane.  bound_ctrl lets the mov fold into the add as
+// its DPP operand; contraction is off so the add is not fused with the product that
+// made a (gfx9 has no DPP form of v_fma).
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float a, float b) {
+#pragma clang fp contract(off)
+    return a + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), CTRL, 0xf, 0xf, true));
+}
+
+// Maps workgroup ids so that consecutive (replica, block) work lands on one XCD:
+// workgroups are dispatched to the 8 XCDs round-robin, each XCD has its own L2.
+__device__ __forceinline__ int xcd_linear_id(int id, int n) {
+    int per = n >> 3, rem = n & 7, x = id & 7, q = id >> 3;
+    return x * per + min(x, rem) + q;
+}
+
+// s_waitcnt vmcnt(0) alone (expcnt and lgkmcnt at their no-wait maxima).  The
+// scheduling barrier keeps the arithmetic before it from being moved past the wait.
+__device__ __forceinline__ void wait_vmem() {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 __device__ __forceinline__ int next_bit(unsigned long long& m) {
     int j = m ? __ffsll((long long)m) - 1 : -1;
     m &= m - 1ull;
@@ -343,22 +451,22 @@ __device__ __forceinline__ void reduce_scatter24(const float (&v)[24], int lane,
     for (int t = 0; t < 3; t++) {
         float keep = b3 ? x[t + 3] : x[t];
         float send = b3 ? x[t] : x[t + 3];
-        float y = keep + dpp_f<0x128>(send);  // row_ror:8 = lane ^ 8 within 16
-        y += dpp_f<0xB1>(y);                  // quad_perm [1,0,3,2] = lane ^ 1
-        y += dpp_f<0x4E>(y);                  // quad_perm [2,3,0,1] = lane ^ 2
-        y += dpp_f<0x141>(y);                 // row_half_mirror: lane i <- 7 - i within 8
-        out[t] = y;
+        float y = dpp_add<0x128>(keep, send);  // row_ror:8 = lane ^ 8 within 16
+        y = dpp_add<0xB1>(y, y);               // quad_perm [1,0,3,2] = lane ^ 1
+        y = dpp_add<0x4E>(y, y);               // quad_perm [2,3,0,1] = lane ^ 2
+        out[t] = dpp_add<0x141>(y, y);         // row_half_mirror: lane i <- 7 - i within 8
     }
 }
 
 // Newton-3 evaluation of one tile: j atoms in groups of 8; their forces are scattered
-// to the slot-ordered fixed-point accumulator F (3 planes of Np).
+// to the slot-ordered fixed-point accumulator F (3 planes of Np).  Slots past the last j
+// atom re-read the first one and are masked off.
 template <bool MINIMG, bool EXCL>
 __device__ __forceinline__ void tile_newton(const float4* __restrict__ P, const float2* __restrict__ Q,
                                             int jbeg, unsigned long long am, float4 pi, float2 qi,
-                                            unsigned long long em, const Geom& g, PairAcc& a,
-                                            unsigned long long* __restrict__ Fr, int Np,
-                                            int lane, int& bad) {
+                                            bool vi, unsigned long long em, const Geom& g,
+                                            PairAcc& a, unsigned long long* __restrict__ Fr,
+                                            int Np, int lane, int& bad) {
     while (am) {
         int jj[8];
 #pragma unroll
@@ -366,17 +474,16 @@ __device__ __forceinline__ void tile_newton(const float4* __restrict__ P, const 
         float v[24];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            v[3 * k] = v[3 * k + 1] = v[3 * k + 2] = 0.f;
-            if (jj[k] >= 0) {
-                float4 pj = P[jbeg + jj[k]];
-                float2 qj = Q[jbeg + jj[k]];
-                float dx, dy, dz;
-                float fr = pair_one<MINIMG, EXCL>(pi, qi, pj, qj, !((em >> jj[k]) & 1ull), g, a,
-                                                  dx, dy, dz);
-                v[3 * k] = -fr * dx;
-                v[3 * k + 1] = -fr * dy;
-                v[3 * k + 2] = -fr * dz;
-            }
+            int j = jj[k] < 0 ? jj[0] : jj[k];
+            float4 pj = P[jbeg + j];
+            float2 qj = Q[jbeg + j];
+            bool incl = vi && jj[k] >= 0;
+            if (EXCL) incl = incl && !((em >> j) & 1ull);
+            float dx, dy, dz;
+            float fr = pair_masked(pi, qi, pj, qj, incl, MINIMG, g, a, dx, dy, dz);
+            v[3 * k] = -fr * dx;
+            v[3 * k + 1] = -fr * dy;
+            v[3 * k + 2] = -fr * dz;
         }
         float f[3];
         reduce_scatter24(v, lane, f);
@@ -386,9 +493,9 @@ __device__ __forceinline__ void tile_newton(const float4* __restrict__ P, const 
         for (int q = 1; q < 8; q++) mine = (k == q) ? jj[q] : mine;
         if ((lane & 7) == 0 && mine >= 0) {
             int s = jbeg + mine;
-            atomicAdd(&Fr[s], (unsigned long long)to_fixed((double)f[0], &bad));
-            atomicAdd(&Fr[Np + s], (unsigned long long)to_fixed((double)f[1], &bad));
-            atomicAdd(&Fr[2 * Np + s], (unsigned long long)to_fixed((double)f[2], &bad));
+            atomicAdd(&Fr[s], fixed_from_float(f[0], &bad));
+            atomicAdd(&Fr[Np + s], fixed_from_float(f[1], &bad));
+            atomicAdd(&Fr[2 * Np + s], fixed_from_float(f[2], &bad));
         }
     }
 }
@@ -407,7 +514,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
     // wave-uniform by construction; readfirstlane lets the compiler keep everything
     // derived from it (replica, block, tile, j index) in scalar registers
     int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int w = blockIdx.x * WPW + wv;
+    int w = xcd_linear_id(blockIdx.x, gridDim.x) * WPW + wv;
     int lane = threadIdx.x & 63;
     if (w >= R * NB) return;
     int r = w / NB;
@@ -466,11 +573,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
         int jbeg = jb * 64;
         if (NEWTON) {
             if (one_image) {
-                if (ex) tile_newton<false, true>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
-                else tile_newton<false, false>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+                if (ex) tile_newton<false, true>(P, Q, jbeg, am, ps, qi, vi, em, g, acc, Fr, Np, lane, bad);
+                else tile_newton<false, false>(P, Q, jbeg, am, ps, qi, vi, em, g, acc, Fr, Np, lane, bad);
             } else {
-                if (ex) tile_newton<true, true>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
-                else tile_newton<true, false>(P, Q, jbeg, am, ps, qi, em, g, acc, Fr, Np, lane, bad);
+                if (ex) tile_newton<true, true>(P, Q, jbeg, am, ps, qi, vi, em, g, acc, Fr, Np, lane, bad);
+                else tile_newton<true, false>(P, Q, jbeg, am, ps, qi, vi, em, g, acc, Fr, Np, lane, bad);
             }
         } else {
             if (one_image) {
@@ -501,6 +608,699 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
     double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
     if (lane == 0) e_pair[(size_t)r * NB + ib] = NEWTON ? es : 0.5 * es;
+}
+
+// ---------------------------------------------------------------------------
+// Block kernel, Newton-3, with LDS-staged j tiles.  Same work split as k_pair<.., true>
+// (one wave per i-block, half tile list, j wave-uniform), but the 64 slot records of
+// each j-block are copied global -> LDS by DMA one tile ahead (no registers, no scalar
+// loads in the pair loop), and the j forces of a tile are summed in LDS and added to
+// the global accumulator once per tile with coalesced atomics.
+//   LDS per wave: two 2 KB tile buffers (separate objects, so reading one never waits
+//   for the copy into the other), 768 B of j-force sums, max_ex*256 B of exclusions.
+//   vmcnt discipline: the only vector memory operations inside the tile loop are the
+//   next tile's copy and the three flush atomics that end each tile, so waiting for
+//   vmcnt <= 3 after the atomics means the copy has landed.
+// ---------------------------------------------------------------------------
+// s_waitcnt vmcnt(3) alone
+__device__ __forceinline__ void wait_vmem3() {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F73);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int WPW, bool COUNT>
+__global__ __launch_bounds__(64 * WPW) void k_pair_n3(
+    int R, int N, int Np, int NB, int maxj, const float4* __restrict__ arec,
+    const int* __restrict__ orig_s, const int* __restrict__ slot_of, const int* __restrict__ tiles,
+    const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
+    const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int max_ex,
+    long long* __restrict__ F, double* __restrict__ e_pair, int* __restrict__ flags,
+    unsigned long long* __restrict__ pair_count, Geom g) {
+    __shared__ float4 tile_a[WPW][128], tile_b[WPW][128];
+    __shared__ float facc[WPW][3][64];
+    extern __shared__ int exs_dyn[];
+    int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int w = xcd_linear_id(blockIdx.x, gridDim.x) * WPW + wv;
+    int lane = threadIdx.x & 63;
+    if (w >= R * NB) return;
+    int r = w / NB;
+    int ib = w - r * NB;
+    const float4* A = arec + (size_t)r * Np * 2;
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    int* exs = exs_dyn + wv * max_ex * 64;
+    int islot = ib * 64 + lane;
+    bool vi = islot < N;
+    float4 pi = A[2 * islot];
+    float4 qi4 = A[2 * islot + 1];
+    float2 qi = make_float2(qi4.x, qi4.y);
+    int nex = 0;
+    if (vi) {
+        int ai = orig_s[(size_t)r * Np + islot];
+        int b0 = ex_start[ai];
+        nex = ex_start[ai + 1] - b0;
+        for (int k = 0; k < nex; k++) exs[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+    }
+    facc[wv][0][lane] = 0.f;
+    facc[wv][1][lane] = 0.f;
+    facc[wv][2][lane] = 0.f;
+    float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
+    double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
+    PairAcc acc;
+    acc.e = 0.0;
+    acc.cnt = 0;
+    int bad = 0;
+    const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
+    const unsigned long long* tmk = tmask + ((size_t)r * NB + ib) * maxj;
+    int nt = ntiles[(size_t)r * NB + ib];
+
+    // copies the 64 slot records (32 B) of j-block jb into tile buffer B: lane L moves
+    // half L&1 of record L>>1 (first instruction) and of record 32 + (L>>1) (second)
+    auto stage = [&](int jb, auto bufc) {
+        constexpr int B = decltype(bufc)::value;
+        float4* dst = B ? tile_b[wv] : tile_a[wv];
+        const char* src = (const char*)(A + 2 * jb * 64) + lane * 16;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(src + 1024),
+                                         (__attribute__((address_space(3))) void*)(dst + 64), 16, 0, 0);
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    auto do_tile = [&](int t, auto bufc) {
+        constexpr int B = decltype(bufc)::value;
+        const float4* buf = B ? tile_b[wv] : tile_a[wv];
+        int jb = tl[t];
+        unsigned long long am = tmk[t];
+        const unsigned long long am0 = am;
+        if (t + 1 < nt) stage(tl[t + 1], std::integral_constant<int, 1 - B>());
+        unsigned long long em = 0;
+        for (int k = 0; k < nex; k++) {
+            int sl = exs[k * 64 + lane];
+            if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+        }
+        if (jb == ib) em |= (2ull << lane) - 1ull;
+        // one periodic shift for the whole tile when both boxes fit in half a box
+        float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
+        float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
+        float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
+        float dcz = 0.5f * (bli.z + bhi.z - blj.z - bhj.z);
+        float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
+              sz = g.bz * rintf(dcz * g.ibz);
+        bool one_image =
+            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
+            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
+            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+        float4 ps = pi;
+        if (one_image) {
+            ps.x -= sx;
+            ps.y -= sy;
+            ps.z -= sz;
+        }
+        const bool ex = __ballot(em != 0ull) != 0ull;
+        acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
+        // all j atoms of the tile, 8 at a time; MI: per-pair minimum image, EX: the tile
+        // holds excluded pairs (both wave-uniform, compile-time inside the loop)
+        auto chunks = [&](auto mic, auto exc) {
+            constexpr bool MI = decltype(mic)::value, EX = decltype(exc)::value;
+            while (am) {
+                int jj[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) jj[k] = next_bit(am);
+                float v[24];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    int j = jj[k] < 0 ? jj[0] : jj[k];
+                    float4 pj = buf[2 * j];
+                    float2 qj = *(const float2*)&buf[2 * j + 1];
+                    bool incl = vi && jj[k] >= 0;
+                    if (EX) incl = incl && !((em >> j) & 1ull);
+                    float dx, dy, dz;
+                    float fr = pair_masked(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
+                    v[3 * k] = -fr * dx;
+                    v[3 * k + 1] = -fr * dy;
+                    v[3 * k + 2] = -fr * dz;
+                }
+                float f[3];
+                reduce_scatter24(v, lane, f);
+                int kq = lane >> 3;
+                int mine = jj[0];
+#pragma unroll
+                for (int q = 1; q < 8; q++) mine = (kq == q) ? jj[q] : mine;
+                // Distinct j atoms per lane and only this wave touches facc, so a plain
+                // read-add-write is exact.  It is written out because the compiler would
+                // otherwise put an s_waitcnt vmcnt(0) (a wait for the tile copy in
+                // flight) in front of any LDS access it cannot separate from that copy;
+                // the loads wait for themselves inside the statement.
+                if ((lane & 7) == 0 && mine >= 0) {
+                    unsigned addr = (unsigned)(size_t)&facc[wv][0][mine];
+                    float o0, o1, o2;
+                    asm volatile(
+                        "ds_read_b32 %0, %3\n\t"
+                        "ds_read_b32 %1, %3 offset:256\n\t"
+                        "ds_read_b32 %2, %3 offset:512\n\t"
+                        "s_waitcnt lgkmcnt(0)"
+                        : "=v"(o0), "=v"(o1), "=v"(o2)
+                        : "v"(addr)
+                        : "memory");
+                    o0 += f[0];
+                    o1 += f[1];
+                    o2 += f[2];
+                    asm volatile(
+                        "ds_write_b32 %0, %1\n\t"
+                        "ds_write_b32 %0, %2 offset:256\n\t"
+                        "ds_write_b32 %0, %3 offset:512"
+                        :
+                        : "v"(addr), "v"(o0), "v"(o1), "v"(o2)
+                        : "memory");
+                }
+            }
+        };
+        using T = std::true_type;
+        using Fa = std::false_type;
+        if (one_image) {
+            if (ex) chunks(Fa(), T());
+            else chunks(Fa(), Fa());
+        } else {
+            if (ex) chunks(T(), T());
+            else chunks(T(), Fa());
+        }
+        ax += (double)acc.fx;
+        ay += (double)acc.fy;
+        az += (double)acc.fz;
+        aqr2 += (double)acc.sqr2;
+        aq += (double)acc.sq;
+        // flush the tile's j forces: lane = j atom, coalesced fixed-point atomics (the
+        // sums are read and cleared the same way as they were added)
+        float fjx, fjy, fjz;
+        {
+            unsigned addr = (unsigned)(size_t)&facc[wv][0][lane];
+            asm volatile(
+                "ds_read_b32 %0, %3\n\t"
+                "ds_read_b32 %1, %3 offset:256\n\t"
+                "ds_read_b32 %2, %3 offset:512\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=v"(fjx), "=v"(fjy), "=v"(fjz)
+                : "v"(addr)
+                : "memory");
+            float z = 0.f;
+            asm volatile(
+                "ds_write_b32 %0, %1\n\t"
+                "ds_write_b32 %0, %1 offset:256\n\t"
+                "ds_write_b32 %0, %1 offset:512"
+                :
+                : "v"(addr), "v"(z)
+                : "memory");
+        }
+        if (!COUNT && ((am0 >> lane) & 1ull)) {
+            int s = jb * 64 + lane;
+            atomicAdd(&Fr[s], fixed_from_float(fjx, &bad));
+            atomicAdd(&Fr[Np + s], fixed_from_float(fjy, &bad));
+            atomicAdd(&Fr[2 * Np + s], fixed_from_float(fjz, &bad));
+        }
+        // the next tile's records have landed (counting mode issues no atomics)
+        if (COUNT) wait_vmem();
+        else wait_vmem3();
+    };
+
+    if (nt > 0) stage(tl[0], std::integral_constant<int, 0>());
+    wait_vmem();  // the first tile, this lane's atom and the block boxes have arrived
+    for (int t = 0; t < nt; t += 2) {
+        do_tile(t, std::integral_constant<int, 0>());
+        if (t + 1 < nt) do_tile(t + 1, std::integral_constant<int, 1>());
+    }
+    if (COUNT) {
+        double c = wave_sum(vi ? (double)acc.cnt : 0.0);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(c + 0.5));
+        return;
+    }
+    if (vi) {  // atomic: the bonded kernel may run concurrently
+        long long fx = to_fixed(ax, &bad), fy = to_fixed(ay, &bad), fz = to_fixed(az, &bad);
+        atomicAdd(&Fr[islot], (unsigned long long)fx);
+        atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
+        atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
+    }
+    double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
+    if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
+    if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped pair kernel (Newton-3, each pair once).  One wave per (replica, 64-slot
+// i-block), split into NG = 64/G groups of G consecutive slots.  Every group streams its
+// own j atoms -- those within rc of the GROUP's box rather than the block's -- so far
+// fewer of the evaluated lane pairs lie outside the cutoff (a 16-atom group's box is
+// about half as wide as a 64-atom block's).
+//
+// Candidate j-blocks (j-block >= i-block) are found by the wave from the block boxes,
+// without a tile list; those that need no periodic shift come first, the rest after
+// them.  For each j-block the 64 lanes test its 64 atoms against every group's box and
+// append the slots that pass to that group's ring buffer in LDS (entry = slot | flags).
+// Full chunks of 8 entries per group are consumed as soon as every group has one; at the
+// end, and when a ring would overflow, partial chunks are consumed with per-lane guards.
+//
+// Within a chunk lane L of a group evaluates ring entry k ^ rot(L) in its register slot
+// k.  With that order the j forces are summed over the group's lanes by halving DPP
+// stages that never select: at each stage a lane keeps slots [0, n/2) and adds its
+// partner's slots [n/2, n), which hold the same j atoms.  The lane then owns the force
+// on the j atom of its register slot 0 and adds it with int64 atomics.
+// ---------------------------------------------------------------------------
+constexpr unsigned RING_SLOT = 0x1FFFFFFFu;
+constexpr unsigned RING_EXCL = 1u << 29;    // pair needs j > i and the exclusion test
+constexpr unsigned RING_MINIMG = 1u << 30;  // pair needs the per-pair minimum image
+
+// Ring order of lane gl (0..G-1) of a group.  G = 16: stage partners lane^8 (bit 3),
+// lane^7 (bits 2,1,0), lane^2 (bit 1), then lane^1 sums equal atoms.  G = 8: lane^7,
+// lane^2, lane^1.  rot flips exactly the bit the stage halves on for each partner.
+template <int G>
+__device__ __forceinline__ int group_rot(int gl) {
+    if (G == 16) {
+        int b3 = (gl >> 3) & 1, b2 = (gl >> 2) & 1, b1 = (gl >> 1) & 1;
+        return 4 * b3 + 2 * b2 + (b1 ^ b2);
+    }
+    int b2 = (gl >> 2) & 1, b1 = (gl >> 1) & 1, b0 = gl & 1;
+    return 4 * b2 + 2 * (b1 ^ b2) + (b0 ^ b2);
+}
+
+// v[3k + c]: component c of the force on ring entry k ^ rot(L).  Returns in out[] the
+// group's sum for ring entry rot(L) (register slot 0).
+template <int G>
+__device__ __forceinline__ void group_reduce24(const float (&v)[24], float (&out)[3]) {
+    float w[12], x[6];
+    if (G == 16) {
+#pragma unroll
+        for (int t = 0; t < 12; t++) w[t] = dpp_add<0x128>(v[t], v[t + 12]);  // row_ror:8
+#pragma unroll
+        for (int t = 0; t < 6; t++) x[t] = dpp_add<0x141>(w[t], w[t + 6]);    // half mirror
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            float y = dpp_add<0x4E>(x[t], x[t + 3]);                          // lane ^ 2
+            out[t] = dpp_add<0xB1>(y, y);                                     // lane ^ 1
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 12; t++) w[t] = dpp_add<0x141>(v[t], v[t + 12]);
+#pragma unroll
+        for (int t = 0; t < 6; t++) x[t] = dpp_add<0x4E>(w[t], w[t + 6]);
+#pragma unroll
+        for (int t = 0; t < 3; t++) out[t] = dpp_add<0xB1>(x[t], x[t + 3]);
+    }
+}
+
+// min / max over the G lanes of a group (same butterfly as the force sum)
+template <int G>
+__device__ __forceinline__ float group_min(float v) {
+    if (G == 16) v = fminf(v, dpp_f<0x128>(v));
+    v = fminf(v, dpp_f<0x141>(v));
+    v = fminf(v, dpp_f<0x4E>(v));
+    return fminf(v, dpp_f<0xB1>(v));
+}
+template <int G>
+__device__ __forceinline__ float group_max(float v) {
+    if (G == 16) v = fmaxf(v, dpp_f<0x128>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    return fmaxf(v, dpp_f<0xB1>(v));
+}
+
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+
+// One chunk: 8 ring entries per group.  SRC 0 / 1: every entry is valid and the records
+// were copied to LDS staging buffer `stage` (record of group q, entry k at
+// stage[2 (8q + k)]); SRC 2: the records are loaded from A and bit k of `valid` says
+// whether register slot k holds an entry.  mi (per-pair minimum image) and ex (entries
+// that need the j > i / exclusion test) are wave-uniform.  The pairs run in the order
+// 0,4,2,6,1,5,3,7 so that the first two halving stages of the j-force sum fold in as
+// soon as both of their slots are done (12 live partial sums instead of 24).  The j
+// force of register slot 0's entry is returned in f.
+template <int G, int SRC>
+__device__ __forceinline__ void grp_chunk(const float4* __restrict__ A,
+                                          const float4* __restrict__ stage, const unsigned (&e)[8],
+                                          unsigned valid, bool mi, bool ex, int gi, int rot,
+                                          float4 pi, float2 qi, int islot, bool vi,
+                                          const int* __restrict__ exs, int nex, const Geom& g,
+                                          PairAcc& a, float (&f)[3]) {
+    constexpr bool STAGED = SRC != 2;
+    constexpr int ord[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+    constexpr int S1 = G == 16 ? 0x128 : 0x141;  // row_ror:8 | row_half_mirror
+    constexpr int S2 = G == 16 ? 0x141 : 0x4E;   // row_half_mirror | lane ^ 2
+    constexpr int S3 = G == 16 ? 0x4E : 0xB1;    // lane ^ 2 | lane ^ 1
+    float4 pj[8];
+    float2 qj[8];
+    auto ld = [&](int k) {
+        if (STAGED) {
+            int ix = 2 * (gi * 8 + (k ^ rot));
+            pj[k] = stage[ix];
+            qj[k] = *(const float2*)&stage[ix + 1];
+        } else {
+            // e << 5 = the slot's byte offset (32-byte records; the flag bits shift out)
+            const char* rec = (const char*)A + (e[k] << 5);
+            pj[k] = *(const float4*)rec;
+            qj[k] = *(const float2*)(rec + 16);
+        }
+    };
+    auto pair = [&](int k, float (&v)[3]) {
+        bool incl = vi;
+        if (!STAGED) incl = incl && ((valid >> k) & 1u);
+        if (ex && (e[k] & RING_EXCL)) {
+            int js = (int)(e[k] & RING_SLOT);
+            bool part = js <= islot;
+            for (int x = 0; x < nex; x++) part |= exs[x * 64] == js;
+            incl = incl && !part;
+        }
+        float dx, dy, dz;
+        float fr = pair_masked(pi, qi, pj[k], qj[k], incl, mi, g, a, dx, dy, dz);
+        v[0] = -fr * dx;
+        v[1] = -fr * dy;
+        v[2] = -fr * dz;
+    };
+    // staged records are read two pairs ahead; global ones (rare partial chunks) just
+    // before their pair, which keeps this path's registers within the staged one's
+    constexpr int AHEAD = STAGED ? 2 : 0;
+#pragma unroll
+    for (int t = 0; t < AHEAD; t++) ld(ord[t]);
+    float w[4][3], x[2][3];
+#pragma unroll
+    for (int t = 0; t < 8; t += 2) {
+        float v0[3], v1[3];
+        if (t + AHEAD < 8) ld(ord[t + AHEAD]);
+        __builtin_amdgcn_sched_barrier(0);
+        pair(ord[t], v0);
+        if (t + 1 + AHEAD < 8) ld(ord[t + 1 + AHEAD]);
+        __builtin_amdgcn_sched_barrier(0);
+        pair(ord[t + 1], v1);
+        const int sl = ord[t];  // slot sl pairs with slot sl + 4: 0, 2, 1, 3
+#pragma unroll
+        for (int c = 0; c < 3; c++) w[sl][c] = dpp_add<S1>(v0[c], v1[c]);
+        if (sl == 2)
+#pragma unroll
+            for (int c = 0; c < 3; c++) x[0][c] = dpp_add<S2>(w[0][c], w[2][c]);
+        if (sl == 3)
+#pragma unroll
+            for (int c = 0; c < 3; c++) x[1][c] = dpp_add<S2>(w[1][c], w[3][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float y = dpp_add<S3>(x[0][c], x[1][c]);
+        f[c] = G == 16 ? dpp_add<0xB1>(y, y) : y;
+    }
+}
+
+
+template <int G, int CAP, int WPW, bool COUNT>
+__global__ __launch_bounds__(64 * WPW) void k_pair_grp(
+    int R, int N, int Np, int NB, const float4* __restrict__ arec,
+    const int* __restrict__ orig_s, const int* __restrict__ slot_of,
+    const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int max_ex,
+    long long* __restrict__ F, double* __restrict__ e_pair, int* __restrict__ flags,
+    unsigned long long* __restrict__ pair_count, float list_r2, Geom g) {
+    constexpr int NG = 64 / G;
+    extern __shared__ unsigned sdm_lds[];
+    // two staging buffers as separate LDS objects: the compiler then knows that reading
+    // one does not have to wait for the copy into the other
+    __shared__ float4 stage_a[WPW][NG * 16], stage_b[WPW][NG * 16];
+    int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int w = xcd_linear_id(blockIdx.x, gridDim.x) * WPW + wv;
+    int lane = threadIdx.x & 63;
+    if (w >= R * NB) return;
+    int r = w / NB;
+    int ib = w - r * NB;
+    // per wave: rings [NG][CAP] | exclusion partners [max_ex][64] | group boxes [NG][2]
+    unsigned* ring = sdm_lds + wv * (NG * CAP + max_ex * 64 + NG * 8);
+    int* exs = (int*)(ring + NG * CAP);
+    const float4* A = arec + (size_t)r * Np * 2;
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    const int gi = lane / G, gl = lane % G;
+    const int rot = group_rot<G>(gl);
+    int islot = ib * 64 + lane;
+    bool vi = islot < N;
+    float4 pi = A[2 * islot];
+    float4 qi4 = A[2 * islot + 1];
+    float2 qi = make_float2(qi4.x, qi4.y);
+    int nex = 0;
+    if (vi) {
+        int ai = orig_s[(size_t)r * Np + islot];
+        int b0 = ex_start[ai];
+        nex = ex_start[ai + 1] - b0;
+        for (int k = 0; k < nex; k++) exs[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+    }
+    // group boxes: every lane knows its own group's, the wave holds all of them uniform
+    float glx = group_min<G>(vi ? pi.x : 3.0e38f), gly = group_min<G>(vi ? pi.y : 3.0e38f),
+          glz = group_min<G>(vi ? pi.z : 3.0e38f);
+    float ghx = group_max<G>(vi ? pi.x : -3.0e38f), ghy = group_max<G>(vi ? pi.y : -3.0e38f),
+          ghz = group_max<G>(vi ? pi.z : -3.0e38f);
+    float gcx = 0.5f * (glx + ghx), gcy = 0.5f * (gly + ghy), gcz = 0.5f * (glz + ghz);
+    float gex = 0.5f * (ghx - glx), gey = 0.5f * (ghy - gly), gez = 0.5f * (ghz - glz);
+    // every group's box in LDS (center, half extent), read back uniformly when appending
+    float4* gbox = (float4*)(exs + max_ex * 64);
+    if (gl == 0) {
+        gbox[2 * gi] = make_float4(gcx, gcy, gcz, 0.f);
+        gbox[2 * gi + 1] = make_float4(gex, gey, gez, 0.f);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // groups with at least one atom (slots are filled in order, so the first lane decides)
+    int ngv = min(NG, (N - ib * 64 + G - 1) / G);
+    float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
+    // ring head, common to every group and a multiple of 8: a chunk takes 8 entries from
+    // every ring, and a ring with fewer has its tail moved up to the new head
+    int h = 0;
+    int tail[NG];
+#pragma unroll
+    for (int q = 0; q < NG; q++) tail[q] = 0;
+    double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
+    PairAcc acc;
+    acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
+    acc.e = 0.0;
+    acc.cnt = 0;
+    int bad = 0;
+    int nchunk = 0;
+    const unsigned rot4 = (unsigned)rot * 4u;
+    const bool own_j = G == 8 || (gl & 1) == 0;  // lane that adds its j atom's force
+    // staging: lane L copies half L&1 of record (L>>1) + 32 i of the chunk (DMA i)
+    int dkoff[NG / 4];
+#pragma unroll
+    for (int i = 0; i < NG / 4; i++) {
+        int ri = i * 32 + (lane >> 1);
+        dkoff[i] = (ri >> 3) * CAP + (ri & 7);
+    }
+    int cur = 0;       // staging buffer of the next full chunk
+    bool pre = false;  // its records are already there
+    // copies the records of ring entries hd..hd+7 of every group into staging buffer b
+    // (global -> LDS without registers; completion is waited for with vmcnt)
+    // copies the records of ring entries hd..hd+7 of every group into staging buffer B
+    // (global -> LDS without registers; completion is waited for with vmcnt)
+    auto stage_chunk = [&](int hd, auto bufc) {
+        constexpr int B = decltype(bufc)::value;
+        float4* dst = B ? stage_b[wv] : stage_a[wv];
+        int hb = hd & (CAP - 1);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < NG / 4; i++) {
+            unsigned ee = ring[dkoff[i] + hb];
+            const char* src = (const char*)A + ((ee << 5) | ((unsigned)(lane & 1) << 4));
+            __builtin_amdgcn_global_load_lds(
+                (const void*)src, (__attribute__((address_space(3))) void*)(dst + i * 64), 16, 0, 0);
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto add_j = [&](unsigned e0, const float (&f)[3]) {
+        int s = (int)(e0 & RING_SLOT);
+        atomicAdd(&Fr[s], fixed_from_float(f[0], &bad));
+        atomicAdd(&Fr[Np + s], fixed_from_float(f[1], &bad));
+        atomicAdd(&Fr[2 * Np + s], fixed_from_float(f[2], &bad));
+    };
+    auto after_chunk = [&]() {
+        h += 8;
+#pragma unroll
+        for (int q = 0; q < NG; q++) tail[q] = max(tail[q], h);
+        if ((++nchunk & 3) == 0) {  // FP32 partial sums of 32 pairs go to FP64
+            ax += (double)acc.fx;
+            ay += (double)acc.fy;
+            az += (double)acc.fz;
+            aqr2 += (double)acc.sqr2;
+            aq += (double)acc.sq;
+            acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
+        }
+    };
+    // full chunk from staging buffer B; stages the next one into the other buffer first
+    // when every ring holds it (next_ok), so its copy overlaps this chunk's arithmetic.
+    // B is a compile-time constant so every LDS access names its buffer.
+    auto consume_full = [&](bool next_ok, auto bufc) {
+        constexpr int B = decltype(bufc)::value;
+        if (!pre) {
+            stage_chunk(h, std::integral_constant<int, B>());
+            wait_vmem();
+        }
+        pre = false;
+        if (next_ok) {
+            stage_chunk(h + 8, std::integral_constant<int, 1 - B>());
+            pre = true;
+        }
+        unsigned base4 = (unsigned)(gi * CAP + (h & (CAP - 1))) * 4u;
+        unsigned e[8];
+        unsigned fl = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            e[k] = *(const unsigned*)((const char*)ring + (((unsigned)k * 4u) ^ rot4) + base4);
+            fl |= e[k];
+        }
+        bool mi = __ballot((fl & RING_MINIMG) != 0u) != 0ull;
+        bool ex = __ballot((fl & RING_EXCL) != 0u) != 0ull;
+        float f[3];
+        grp_chunk<G, B>(A, B ? stage_b[wv] : stage_a[wv], e, 0xFFu, mi, ex, gi, rot, pi, qi,
+                        islot, vi, exs + lane, nex, g, acc, f);
+        wait_vmem();  // the next chunk's records have landed; older atomics are done
+        if (own_j) add_j(e[0], f);
+        cur ^= 1;
+        after_chunk();
+    };
+    // partial chunk (fewer than 8 entries in some ring): records loaded directly
+    auto consume_partial = [&]() {
+        pre = false;
+        int t = tail[0];
+#pragma unroll
+        for (int q = 1; q < NG; q++) t = (gi == q) ? tail[q] : t;
+        int n = t - h;
+        unsigned base4 = (unsigned)(gi * CAP + (h & (CAP - 1))) * 4u;
+        unsigned e[8];
+        unsigned valid = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            bool ok = (k ^ rot) < n;
+            unsigned v = *(const unsigned*)((const char*)ring + (((unsigned)k * 4u) ^ rot4) + base4);
+            e[k] = ok ? v : (unsigned)islot;  // unused slot: own record
+            valid |= ok ? (1u << k) : 0u;
+        }
+        float f[3];
+        grp_chunk<G, 2>(A, nullptr, e, valid, true, true, gi, rot, pi, qi, islot, vi, exs + lane, nex, g, acc, f);
+        if (own_j && (valid & 1u)) add_j(e[0], f);
+        after_chunk();
+    };
+    auto min_avail = [&]() {
+        int m = tail[0];
+#pragma unroll
+        for (int q = 1; q < NG; q++) m = min(m, tail[q]);
+        return m - h;
+    };
+    auto max_avail = [&]() {
+        int m = tail[0];
+#pragma unroll
+        for (int q = 1; q < NG; q++) m = max(m, tail[q]);
+        return m - h;
+    };
+
+    // candidate j-blocks of one pass among blocks base..base+63 (pass 0: the plain
+    // difference is the minimum image for every pair of the two boxes; pass 1: the rest)
+    auto candidates = [&](int pass, int base) {
+        int jb = base + lane;
+        bool cand = false;
+        if (jb < NB && jb >= ib) {
+            float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
+            float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
+            float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
+            float dcz = 0.5f * (bli.z + bhi.z - blj.z - bhj.z);
+            bool plain = fabsf(dcx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
+                         fabsf(dcy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
+                         fabsf(dcz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+            cand = (plain ? 0 : 1) == pass && bbox_gap2(bli, bhi, blj, bhj, g) < list_r2;
+        }
+        return __ballot(cand);
+    };
+    // appends the atoms of j-block jb that lie within rc of each group's box
+    auto append = [&](int jb, int pass) {
+        __builtin_amdgcn_wave_barrier();
+        int s = jb * 64 + lane;
+        bool vj = s < N;
+        float4 pj = A[2 * min(s, N - 1)];
+        bool mine = jb == ib;
+        for (int k = 0; k < nex; k++) mine |= (exs[k * 64 + lane] >> 6) == jb;
+        unsigned fl = (__ballot(mine) != 0ull ? RING_EXCL : 0u) | (pass ? RING_MINIMG : 0u);
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+            float4 c = gbox[2 * q], h = gbox[2 * q + 1];
+            float dx = pj.x - c.x, dy = pj.y - c.y, dz = pj.z - c.z;
+            if (pass) {
+                dx -= g.bx * rintf(dx * g.ibx);
+                dy -= g.by * rintf(dy * g.iby);
+                dz -= g.bz * rintf(dz * g.ibz);
+            }
+            float ux = fmaxf(fabsf(dx) - h.x, 0.f), uy = fmaxf(fabsf(dy) - h.y, 0.f),
+                  uz = fmaxf(fabsf(dz) - h.z, 0.f);
+            bool in = vj && q < ngv && ux * ux + uy * uy + uz * uz < list_r2;
+            if (jb == ib) in = in && s > ib * 64 + q * G;  // j must follow an i of the group
+            unsigned long long mq = __ballot(in);
+            if (in) ring[q * CAP + ((tail[q] + lane_rank(mq)) & (CAP - 1))] = (unsigned)s | fl;
+            tail[q] += __popcll(mq);
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    // Fill first (every ring 16 deep, so a full chunk can stage the next one), then full
+    // chunks; partial chunks only when a ring would overflow or no tile is left.
+    int pass = 0, base = ib & ~63;
+    unsigned long long tm = candidates(0, base);
+    bool tiles_left = true;
+    // Everything loaded so far (this lane's atom, the block boxes) has arrived: without
+    // this wait the compiler would place one in the loop at the first use of each such
+    // value, and it would then also wait for the staging copy in flight every chunk.
+    wait_vmem();
+    while (true) {
+        int mn = min_avail(), mx = max_avail();
+        if (tiles_left && mn < 16 && mx <= CAP - 64) {
+            while (tm == 0ull && tiles_left) {
+                base += 64;
+                if (base >= NB) {
+                    if (pass == 1) {
+                        tiles_left = false;
+                        break;
+                    }
+                    pass = 1;
+                    base = ib & ~63;
+                }
+                tm = candidates(pass, base);
+            }
+            if (!tiles_left) continue;
+            int jb = base + __ffsll((long long)tm) - 1;
+            tm &= tm - 1ull;
+            append(jb, pass);
+            continue;
+        }
+        if (mn >= 8) {
+            if (cur == 0) consume_full(mn >= 16, std::integral_constant<int, 0>());
+            else consume_full(mn >= 16, std::integral_constant<int, 1>());
+            continue;
+        }
+        if (mx > 0) {
+            consume_partial();
+            continue;
+        }
+        break;
+    }
+    ax += (double)acc.fx;
+    ay += (double)acc.fy;
+    az += (double)acc.fz;
+    aqr2 += (double)acc.sqr2;
+    aq += (double)acc.sq;
+    if (COUNT) {
+        double c = wave_sum(vi ? (double)acc.cnt : 0.0);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(c + 0.5));
+        return;
+    }
+    if (vi) {
+        long long fx = to_fixed(ax, &bad), fy = to_fixed(ay, &bad), fz = to_fixed(az, &bad);
+        atomicAdd(&Fr[islot], (unsigned long long)fx);
+        atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
+        atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
+    }
+    double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
+    if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
+    if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
 }
 
 // ---------------------------------------------------------------------------
