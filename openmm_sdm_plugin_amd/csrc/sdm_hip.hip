@@ -90,7 +90,7 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
-    sdm_tuning tune{2, 0, 1, 1, 16};
+    sdm_tuning tune{1, 0, 1, 1, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool components_valid = false;
@@ -222,7 +222,7 @@ void enqueue_sort(sdm_context* c) {
                                                 c->par_s, c->arec, c->F, c->g);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
     if (c->tune.pair_group == 0)  // the grouped kernel finds its j-blocks itself
-        k_tiles<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo,
+        k_tiles<<<dim3(c->NB, R), 256, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo,
                                               c->bb_hi, c->tiles, c->tmask, c->ntiles,
                                               c->list_r2, c->g, c->tune.pair_newton3, c->flags);
 }

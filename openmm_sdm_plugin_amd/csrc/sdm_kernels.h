@@ -173,6 +173,10 @@ __device__ __forceinline__ float bbox_gap2(float4 lo_i, float4 hi_i, float4 lo_j
     return gx * gx + gy * gy + gz * gz;
 }
 
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 // min / max over the 8 lanes of each group of 8 (lane ^ 7, lane ^ 2, lane ^ 1 via DPP)
 __device__ __forceinline__ float min8(float v) {
     v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, true)));
@@ -191,22 +195,30 @@ __device__ __forceinline__ float max8(float v) {
 // (atoms outside all of them cannot have an in-cutoff partner in the i-block; the
 // union of the small boxes is a much tighter region than the block's box).  Half list:
 // in the diagonal tile an atom only counts for groups that start below it.
-__global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
-                                              const float4* __restrict__ posq_s,
-                                              const float4* __restrict__ bb_lo,
-                                              const float4* __restrict__ bb_hi,
-                                              int* __restrict__ tiles,
-                                              unsigned long long* __restrict__ tmask,
-                                              int* __restrict__ ntiles, float list_r2, Geom g,
-                                              int half, int* __restrict__ flags) {
+// Four waves per i-block: the candidate j-blocks (box test) are listed in LDS, each
+// wave computes the atom masks of every fourth candidate, and wave 0 compacts the
+// non-empty ones in candidate order.
+constexpr int TILE_CAND = 256;  // candidates per round
+__global__ __launch_bounds__(256) void k_tiles(int N, int Np, int NB, int maxj,
+                                               const float4* __restrict__ posq_s,
+                                               const float4* __restrict__ bb_lo,
+                                               const float4* __restrict__ bb_hi,
+                                               int* __restrict__ tiles,
+                                               unsigned long long* __restrict__ tmask,
+                                               int* __restrict__ ntiles, float list_r2, Geom g,
+                                               int half, int* __restrict__ flags) {
     __shared__ float4 sub[8][2];
-    int r = blockIdx.y, ib = blockIdx.x, lane = threadIdx.x;
+    __shared__ int cand[TILE_CAND];
+    __shared__ unsigned long long cmk[TILE_CAND];
+    __shared__ int ncand_s, base_s;
+    int r = blockIdx.y, ib = blockIdx.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const float4* lo = bb_lo + (size_t)r * NB;
     const float4* hi = bb_hi + (size_t)r * NB;
     const float4* P = posq_s + (size_t)r * Np;
     float4 li = lo[ib], hi_i = hi[ib];
-    // the eight group boxes (empty groups past N get an inverted box and never match)
-    {
+    // the eight group boxes (empty groups past N get a negative extent and never match)
+    if (wave == 0) {
         int is = ib * 64 + lane;
         bool v = is < N;
         float4 p = P[min(is, N - 1)];
@@ -218,48 +230,114 @@ __global__ __launch_bounds__(64) void k_tiles(int N, int Np, int NB, int maxj,
             sub[lane >> 3][1] = any ? make_float4(0.5f * (hx - lx), 0.5f * (hy - ly), 0.5f * (hz - lz), 0.f)
                                     : make_float4(-1.0e30f, -1.0e30f, -1.0e30f, 0.f);
         }
-        __syncthreads();
     }
-    int* out = tiles + ((size_t)r * NB + ib) * maxj;
-    unsigned long long* mout = tmask + ((size_t)r * NB + ib) * maxj;
-    int count = 0;
-    for (int base = half ? (ib & ~63) : 0; base < NB; base += 64) {
-        int jb = base + lane;
-        bool ok = false;
-        if (jb < NB && (!half || jb >= ib)) ok = bbox_gap2(li, hi_i, lo[jb], hi[jb], g) < list_r2;
-        unsigned long long m = __ballot(ok);
-        while (m) {
-            int k = __ffsll((long long)m) - 1;
-            m &= m - 1ull;
-            int jb2 = base + k;
-            int s = jb2 * 64 + lane;
-            float4 p = P[s];
-            bool in = false;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                float4 c = sub[q][0], h = sub[q][1];
-                float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+    // atom mask of candidate j-block jb2 (lane = atom)
+    auto mask_of = [&](int jb2) {
+        int s = jb2 * 64 + lane;
+        float4 p = P[s];
+        // one periodic shift for the whole tile when both blocks' boxes fit in half a
+        // box (as in the pair kernel); otherwise the minimum image per group
+        float4 lj = lo[jb2], hj = hi[jb2];
+        float dcx = 0.5f * (li.x + hi_i.x - lj.x - hj.x);
+        float dcy = 0.5f * (li.y + hi_i.y - lj.y - hj.y);
+        float dcz = 0.5f * (li.z + hi_i.z - lj.z - hj.z);
+        float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
+              sz = g.bz * rintf(dcz * g.ibz);
+        bool one_image =
+            fabsf(dcx - sx) + 0.5f * (hi_i.x - li.x + hj.x - lj.x) < 0.5f * g.bx - 1e-3f &&
+            fabsf(dcy - sy) + 0.5f * (hi_i.y - li.y + hj.y - lj.y) < 0.5f * g.by - 1e-3f &&
+            fabsf(dcz - sz) + 0.5f * (hi_i.z - li.z + hj.z - lj.z) < 0.5f * g.bz - 1e-3f;
+        if (one_image) {
+            p.x += sx;
+            p.y += sy;
+            p.z += sz;
+        }
+        // groups whose box is within rc of the j-block's box (lane q tests group q);
+        // only those are tested atom by atom
+        unsigned long long rel;
+        {
+            float4 c = sub[lane & 7][0], h = sub[lane & 7][1];
+            float dx = 0.5f * (lj.x + hj.x) - c.x, dy = 0.5f * (lj.y + hj.y) - c.y,
+                  dz = 0.5f * (lj.z + hj.z) - c.z;
+            if (one_image) {
+                dx += sx;
+                dy += sy;
+                dz += sz;
+            } else {
                 dx -= g.bx * rintf(dx * g.ibx);
                 dy -= g.by * rintf(dy * g.iby);
                 dz -= g.bz * rintf(dz * g.ibz);
-                float gx = fmaxf(fabsf(dx) - h.x, 0.f), gy = fmaxf(fabsf(dy) - h.y, 0.f),
-                      gz = fmaxf(fabsf(dz) - h.z, 0.f);
-                bool inq = gx * gx + gy * gy + gz * gz < list_r2 && h.x >= 0.f;
-                if (half && jb2 == ib) inq = inq && s > ib * 64 + q * 8;
-                in = in || inq;
             }
-            in = in && (s < N);
-            unsigned long long am = __ballot(in);
-            if (am != 0ull) {
-                if (lane == 0 && count < maxj) {
-                    out[count] = jb2;
-                    mout[count] = am;
-                }
-                count++;
+            float gx = fmaxf(fabsf(dx) - h.x - 0.5f * (hj.x - lj.x), 0.f);
+            float gy = fmaxf(fabsf(dy) - h.y - 0.5f * (hj.y - lj.y), 0.f);
+            float gz = fmaxf(fabsf(dz) - h.z - 0.5f * (hj.z - lj.z), 0.f);
+            rel = __ballot(lane < 8 && h.x >= 0.f && gx * gx + gy * gy + gz * gz < list_r2);
+        }
+        bool in = false;
+        while (rel) {
+            int q = __ffsll((long long)rel) - 1;
+            rel &= rel - 1ull;
+            float4 c = sub[q][0], h = sub[q][1];
+            float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+            if (!one_image) {
+                dx -= g.bx * rintf(dx * g.ibx);
+                dy -= g.by * rintf(dy * g.iby);
+                dz -= g.bz * rintf(dz * g.ibz);
+            }
+            float gx = fmaxf(fabsf(dx) - h.x, 0.f), gy = fmaxf(fabsf(dy) - h.y, 0.f),
+                  gz = fmaxf(fabsf(dz) - h.z, 0.f);
+            bool inq = gx * gx + gy * gy + gz * gz < list_r2;
+            if (half && jb2 == ib) inq = inq && s > ib * 64 + q * 8;
+            in = in || inq;
+        }
+        return __ballot(in && (s < N));
+    };
+    int* out = tiles + ((size_t)r * NB + ib) * maxj;
+    unsigned long long* mout = tmask + ((size_t)r * NB + ib) * maxj;
+    int count = 0;    // wave 0's running output count
+    int base = half ? (ib & ~63) : 0;
+    while (base < NB) {
+        // list up to TILE_CAND candidates (wave 0), continuing from base
+        if (wave == 0) {
+            int nc = 0;
+            while (base < NB && nc + 64 <= TILE_CAND) {
+                int jb = base + lane;
+                bool ok = false;
+                if (jb < NB && (!half || jb >= ib)) ok = bbox_gap2(li, hi_i, lo[jb], hi[jb], g) < list_r2;
+                unsigned long long m = __ballot(ok);
+                if (ok) cand[nc + lane_rank(m)] = jb;
+                nc += __popcll(m);
+                base += 64;
+            }
+            if (lane == 0) {
+                ncand_s = nc;
+                base_s = base;
             }
         }
+        __syncthreads();
+        int nc = ncand_s;
+        base = base_s;  // every wave continues from wave 0's position (uniform loop)
+        for (int c = wave; c < nc; c += 4) {
+            unsigned long long am = mask_of(cand[c]);
+            if (lane == 0) cmk[c] = am;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                int c = c0 + lane;
+                unsigned long long am = c < nc ? cmk[c] : 0ull;
+                unsigned long long keep = __ballot(am != 0ull);
+                int pos = count + lane_rank(keep);
+                if (am != 0ull && pos < maxj) {
+                    out[pos] = cand[c];
+                    mout[pos] = am;
+                }
+                count += __popcll(keep);
+            }
+        }
+        __syncthreads();
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         ntiles[(size_t)r * NB + ib] = min(count, maxj);
         if (count > maxj) atomicOr(&flags[r], 4);
     }
@@ -925,9 +1003,6 @@ __device__ __forceinline__ float group_max(float v) {
     return fmaxf(v, dpp_f<0xB1>(v));
 }
 
-__device__ __forceinline__ int lane_rank(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
 
 
 // One chunk: 8 ring entries per group.  SRC 0 / 1: every entry is valid and the records
