@@ -39,12 +39,15 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
+    ap.add_argument("--profile-pair", type=int, default=0,
+                    help="profiling mode: the pair kernel alone N times (the launches the "
+                         "roofline figure times), no other legs")
     return ap.parse_args()
 
 
 def read_traffic():
     """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "k_pair_config3_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k_pair_n3_config3_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -101,13 +104,27 @@ def main():
         ctx.set_positions(r, frames[r].positions)
         ctx.set_velocities(r, frames[r].velocities)
     ctx.set_u_history(100, 4096)
-    if a.profile:
+    if a.profile or a.profile_pair:
         # one launch per kernel (no graph) so a profiler attributes every dispatch
         ctx.set_tuning(use_graph=0)
-        ctx.step(a.profile)
+        if a.profile_pair:
+            ms = ctx.time_pair_kernel(a.profile_pair)
+            print(json.dumps({"pair_kernel_ms_events": ms}))
+        else:
+            ctx.step(a.profile)
         e = ctx.energies()
-        print(json.dumps({"profile_steps": a.profile, "step": e[0]["step"],
-                          "flags": [x["flags"] for x in e], "tuning": ctx.get_tuning()}))
+        # calibration for the HBM-traffic counters: a fill that writes 64 MiB and a sum
+        # that reads the same 64 MiB (plain streaming kernels of known size); a second
+        # 64 MiB fill in between evicts the first buffer from the L2s (32 MiB in all)
+        src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        evict = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        tot = float(src.sum())
+        torch.cuda.synchronize()
+        del evict
+        print(json.dumps({"profile_steps": a.profile or a.profile_pair, "step": e[0]["step"],
+                          "flags": [x["flags"] for x in e], "tuning": ctx.get_tuning(),
+                          "calibration_bytes": src.numel() * 4, "calibration_sum": tot}))
         return
     tuning = ctx.autotune(20) if not a.no_autotune else ctx.get_tuning()
 
@@ -176,7 +193,7 @@ def main():
         "batched_over_plain": agg / single,
         "all_finite_no_flags": ok,
         "roofline": {
-            "bound": "valu", "kernel": "k_pair", "achieved": achieved,
+            "bound": "valu", "kernel": "k_pair_n3", "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic, "pair_kernel_ms": pair_ms,
             "pairs_per_launch": pairs, "flop_per_pair": FLOP_PER_PAIR,

@@ -206,8 +206,9 @@ int sdm_get_u_history(sdm_context* ctx, int32_t replica, double* out, int32_t ca
    non-excluded pairs with r < cutoff at x (p0) and of pairs with d_i != d_j and
    r < cutoff at x+d (pdelta). */
 int sdm_count_pairs(sdm_context* ctx, int64_t* p0, int64_t* pdelta /* [n_replicas] */);
-/* Average duration (ms) of the nonbonded pair kernel over nsteps eager steps, timed
-   with HIP events on the context's stream. */
+/* Average duration (ms) of the nonbonded pair kernel: nsteps launches back to back, each
+   after a spatial sort and with nothing running beside it, timed with HIP events on the
+   context's stream.  Positions do not advance; forces must be recomputed afterwards. */
 int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 
 /* Launch-configuration search (project-written autotuner): times each candidate on

@@ -922,21 +922,35 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
 
 int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
     if (!c || !avg_ms || nsteps <= 0) return fail(SDM_ERR_INVALID, "bad argument");
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    // The pair kernel alone (spatial sort, then the pair kernel; no bonded or
+    // displaced-state work running beside it), nsteps times back to back without
+    // synchronisation, each launch between its own pair of events on the context's
+    // stream.  Positions do not advance; the force accumulator is left incomplete.
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps, nullptr);
+    int rc = SDM_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(SDM_ERR_HIP, "hipEventCreate failed");
     double total = 0.0;
-    for (int s = 0; s < nsteps; s++) {
-        enqueue_step(c, e0, e1);
-        CHK(check_launch());
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        total += ms;
+    if (rc == SDM_OK) {
+        for (int s = 0; s < nsteps; s++) {
+            enqueue_sort(c);
+            (void)hipEventRecord(ev[2 * s], c->stream);
+            enqueue_pair(c, false);
+            (void)hipEventRecord(ev[2 * s + 1], c->stream);
+        }
+        rc = check_launch();
+        if (rc == SDM_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(SDM_ERR_HIP, "hipStreamSynchronize failed");
+        for (int s = 0; s < nsteps && rc == SDM_OK; s++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]) != hipSuccess)
+                rc = fail(SDM_ERR_HIP, "hipEventElapsedTime failed");
+            total += ms;
+        }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc != SDM_OK) return rc;
     *avg_ms = total / nsteps;
     c->components_valid = false;
     return SDM_OK;
