@@ -360,7 +360,8 @@ __global__ __launch_bounds__(256) void k_tiles(int N, int Np, int NB, int maxj,
 struct PairAcc {
     float fx, fy, fz;  // per-tile force on the lane's i atom
     float sqr2, sq;    // per-tile sums of qq r^2 and qq (reaction-field energy terms)
-    double e;          // LJ + qq/r energy, FP64 per pair (net energies cancel strongly)
+    double e;          // LJ + qq/r energy in FP64 (net energies cancel strongly)
+    float ef;          // the same over the current chunk of 8 pairs, added to e per chunk
     int cnt;
 };
 
@@ -436,13 +437,13 @@ __device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, fl
     a.fx += fr * dx;
     a.fy += fr * dy;
     a.fz += fr * dz;
-    a.e += (double)ee;
+    a.ef += ee;
     a.sqr2 += qqm * r2;
     a.sq += qqm;
     a.cnt += (int)(m & 1u);
     // pin the sums here: otherwise the compiler sinks every pair's accumulation to the
     // end of the chunk and keeps all eight pairs' operands live (VGPR pressure)
-    asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz), "+v"(a.e), "+v"(a.sqr2), "+v"(a.sq));
+    asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz), "+v"(a.ef), "+v"(a.sqr2), "+v"(a.sq));
     return fr;
 }
 
@@ -565,6 +566,8 @@ __device__ __forceinline__ void tile_newton(const float4* __restrict__ P, const 
         }
         float f[3];
         reduce_scatter24(v, lane, f);
+        a.e += (double)a.ef;  // FP32 over 8 pairs, FP64 beyond
+        a.ef = 0.f;
         int k = lane >> 3;
         int mine = jj[0];
 #pragma unroll
@@ -615,6 +618,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
     double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
     PairAcc acc;
     acc.e = 0.0;
+    acc.ef = 0.f;
     acc.cnt = 0;
     int bad = 0;
     const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
@@ -747,6 +751,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
     double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
     PairAcc acc;
     acc.e = 0.0;
+    acc.ef = 0.f;
     acc.cnt = 0;
     int bad = 0;
     const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
@@ -823,6 +828,8 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                 }
                 float f[3];
                 reduce_scatter24(v, lane, f);
+                acc.e += (double)acc.ef;  // FP32 over 8 pairs, FP64 beyond
+                acc.ef = 0.f;
                 int kq = lane >> 3;
                 int mine = jj[0];
 #pragma unroll
@@ -1084,6 +1091,8 @@ __device__ __forceinline__ void grp_chunk(const float4* __restrict__ A,
         float y = dpp_add<S3>(x[0][c], x[1][c]);
         f[c] = G == 16 ? dpp_add<0xB1>(y, y) : y;
     }
+    a.e += (double)a.ef;  // FP32 over 8 pairs, FP64 beyond
+    a.ef = 0.f;
 }
 
 
@@ -1152,6 +1161,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_grp(
     PairAcc acc;
     acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
     acc.e = 0.0;
+    acc.ef = 0.f;
     acc.cnt = 0;
     int bad = 0;
     int nchunk = 0;
