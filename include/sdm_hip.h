@@ -132,6 +132,10 @@ typedef struct {
                                      j stream); 16 or 8: grouped kernel (the wave's i-block
                                      split into groups of 16 or 8 atoms, each with its own j
                                      stream, each pair once) */
+    int32_t sort_interval;        /* steps between full spatial sorts (1: every step); the
+                                     steps in between keep the slot order and only refresh
+                                     the coordinates and block boxes.  Results differ only
+                                     in the FP32 summation order of the pair energies. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -90,9 +90,11 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
-    sdm_tuning tune{1, 0, 1, 1, 0};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    sdm_tuning tune{1, 0, 1, 1, 0, 20};
+    // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
+    int since_sort = 0;
+    hipGraph_t graph = nullptr, graph_keep = nullptr;  // step with / without a full sort
+    hipGraphExec_t gexec = nullptr, gexec_keep = nullptr;
     bool components_valid = false;
     std::vector<void*> allocs;
 };
@@ -106,9 +108,13 @@ int dalloc(sdm_context* c, T** p, size_t n) {
     hipError_t e = hipMalloc(&q, n * sizeof(T));
     if (e != hipSuccess)
         return fail(SDM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(q, 0, n * sizeof(T));
+    // zero-filled on the context's own stream and waited for: the legacy null stream is
+    // not ordered with the context's non-blocking streams, and a reused allocation holds
+    // stale values (counters that must start at zero would give out-of-range offsets)
+    e = hipMemsetAsync(q, 0, n * sizeof(T), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess)
-        return fail(SDM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+        return fail(SDM_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     c->allocs.push_back(q);
     *p = (T*)q;
     return SDM_OK;
@@ -133,8 +139,20 @@ int download(sdm_context* c, T* dst, const T* src, size_t n) {
 void drop_graph(sdm_context* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
+    if (c->gexec_keep) (void)hipGraphExecDestroy(c->gexec_keep);
+    if (c->graph_keep) (void)hipGraphDestroy(c->graph_keep);
+    c->gexec = c->gexec_keep = nullptr;
+    c->graph = c->graph_keep = nullptr;
+}
+
+// Whether an evaluation sorts (true) or keeps the slot order of the last sort.  Steps
+// sort every tune.sort_interval steps; other evaluations (energies, pair counts, kernel
+// timing) sort only a stale order and leave the count alone, so the orders that a run
+// of steps sees do not depend on them.
+bool take_order(sdm_context* c, bool step) {
+    bool full = c->since_sort == 0 || (step && c->since_sort >= c->tune.sort_interval);
+    if (step) c->since_sort = full ? 1 : c->since_sort + 1;
+    return full;
 }
 
 // Hilbert index of a cell (Skilling, "Programming the Hilbert curve", AIP Conf. Proc.
@@ -197,6 +215,7 @@ int setup_cells(sdm_context* c, int per_dim) {
     } else {
         // counters must start at zero for the new grid
         HIPCHK(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->R * c->NC_alloc, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     CHK(upload(c, c->cellrank, rank.data(), NC));
     c->ncx = n[0];
@@ -209,10 +228,15 @@ int setup_cells(sdm_context* c, int per_dim) {
 
 dim3 g2(int n, int per, int R) { return dim3((unsigned)((n + per - 1) / per), (unsigned)R); }
 
-// spatial sort and tile lists for every replica
-void enqueue_sort(sdm_context* c) {
+// slot order (a full spatial sort, or the last sort's order refreshed), block boxes and
+// tile lists for every replica
+void enqueue_sort(sdm_context* c, bool full) {
     hipStream_t s = c->stream;
     int N = c->N, Np = c->Np, R = c->R;
+    if (!full) {
+        k_refresh<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->x64, c->orig_s, c->qs, c->g,
+                                                c->posq_s, c->arec, c->F, c->bb_lo, c->bb_hi);
+    } else {
     k_bin<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->x64, c->key, c->ccount, c->cellrank, c->g,
                                         c->ncx, c->ncy, c->ncz, c->NC);
     k_scan<<<R, 1024, 0, s>>>(c->NC, c->ccount, c->cstart, c->cfill);
@@ -221,6 +245,7 @@ void enqueue_sort(sdm_context* c) {
                                                 c->x64, c->qs, c->sh, c->se, c->posq_s,
                                                 c->par_s, c->arec, c->F, c->g);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
+    }
     if (c->tune.pair_group == 0)  // the grouped kernel finds its j-blocks itself
         k_tiles<<<dim3(c->NB, R), 256, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo,
                                               c->bb_hi, c->tiles, c->tmask, c->ntiles,
@@ -313,11 +338,11 @@ void enqueue_pair(sdm_context* c, bool count) {
 // work forks: the pair kernel on the main stream, the bonded terms and the displaced-state
 // chain on two side streams; they join before the scalar stage.  pe0/pe1 (optional) are
 // recorded around the pair kernel.
-void enqueue_eval(sdm_context* c, bool advance, bool components, hipEvent_t pe0 = nullptr,
-                  hipEvent_t pe1 = nullptr) {
+void enqueue_eval(sdm_context* c, bool advance, bool components, bool full,
+                  hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
     hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
-    enqueue_sort(c);
+    enqueue_sort(c, full);
     (void)hipEventRecord(c->ev_fork, s);
     (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
     (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
@@ -359,8 +384,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, hipEvent_t pe0 
                                                        c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
-void enqueue_step(sdm_context* c, hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
-    enqueue_eval(c, true, false, pe0, pe1);
+void enqueue_step(sdm_context* c, bool full, hipEvent_t pe0 = nullptr,
+                  hipEvent_t pe1 = nullptr) {
+    enqueue_eval(c, true, false, full, pe0, pe1);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
                                                            c->v64, c->rec, c->state, c->slot_of);
 }
@@ -371,13 +397,14 @@ int check_launch() {
     return SDM_OK;
 }
 
-int build_graph(sdm_context* c) {
-    drop_graph(c);
+int build_graph(sdm_context* c, bool full) {
+    hipGraph_t& graph = full ? c->graph : c->graph_keep;
+    hipGraphExec_t& gexec = full ? c->gexec : c->gexec_keep;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c);
-    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    enqueue_step(c, full);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     return SDM_OK;
 }
 
@@ -690,6 +717,7 @@ int sdm_set_time(sdm_context* c, int32_t replica, double time, int64_t step) {
 
 int sdm_set_nonbonded_params(sdm_context* c, const double* q, const double* sig, const double* eps) {
     if (!c || !q || !sig || !eps) return fail(SDM_ERR_INVALID, "null argument");
+    c->since_sort = 0;  // the slot-ordered parameter copies are rebuilt by the next sort
     return upload_nb_params(c, q, sig, eps);
 }
 
@@ -716,7 +744,10 @@ static int get_vec(sdm_context* c, const double* base, int32_t r, double* xyz) {
     return SDM_OK;
 }
 
-int sdm_set_positions(sdm_context* c, int32_t r, const double* xyz) { return set_vec(c, c->x64, r, xyz); }
+int sdm_set_positions(sdm_context* c, int32_t r, const double* xyz) {
+    if (c) c->since_sort = 0;  // the slot order is re-sorted from the new positions
+    return set_vec(c, c->x64, r, xyz);
+}
 int sdm_get_positions(sdm_context* c, int32_t r, double* xyz) { return get_vec(c, c->x64, r, xyz); }
 int sdm_set_velocities(sdm_context* c, int32_t r, const double* xyz) { return set_vec(c, c->v64, r, xyz); }
 int sdm_get_velocities(sdm_context* c, int32_t r, double* xyz) { return get_vec(c, c->v64, r, xyz); }
@@ -741,6 +772,7 @@ extern "C" {
 
 int sdm_set_positions_device(sdm_context* c, const double* d_xyz) {
     if (!c || !d_xyz) return fail(SDM_ERR_INVALID, "null argument");
+    c->since_sort = 0;
     k_aos_to_soa<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, d_xyz, c->x64);
     CHK(check_launch());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -757,7 +789,7 @@ int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
 
 int sdm_compute(sdm_context* c) {
     if (!c) return fail(SDM_ERR_INVALID, "null context");
-    enqueue_eval(c, false, false);
+    enqueue_eval(c, false, false, take_order(c, false));
     c->components_valid = false;
     return check_launch();
 }
@@ -767,11 +799,14 @@ int sdm_step(sdm_context* c, int32_t nsteps) {
     if (nsteps < 0) return fail(SDM_ERR_INVALID, "nsteps must be >= 0");
     c->components_valid = false;
     if (nsteps == 0) return SDM_OK;
-    if (c->tune.use_graph) {
-        if (!c->gexec) CHK(build_graph(c));
-        for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-    } else {
-        for (int s = 0; s < nsteps; s++) enqueue_step(c);
+    for (int s = 0; s < nsteps; s++) {
+        bool full = take_order(c, true);
+        if (c->tune.use_graph) {
+            if (!(full ? c->gexec : c->gexec_keep)) CHK(build_graph(c, full));
+            HIPCHK(hipGraphLaunch(full ? c->gexec : c->gexec_keep, c->stream));
+        } else {
+            enqueue_step(c, full);
+        }
     }
     return check_launch();
 }
@@ -809,7 +844,7 @@ int sdm_compute_components(sdm_context* c) {
     }
     k_fill_ll<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->Fb_sep, n3);
     k_fill_d<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->dFd, n3);
-    enqueue_eval(c, false, true);
+    enqueue_eval(c, false, true, take_order(c, false));
     if (c->L > 0)
         k_scatter_dF<<<dim3(c->maxc + c->LB, c->R), 64, 0, c->stream>>>(
             c->N, c->Np, c->L, c->maxc, c->lig, c->cls, c->orig_s, c->cand, c->ncand, c->dB,
@@ -895,7 +930,7 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
     size_t n = 2 * (size_t)c->R;
     hipStream_t s = c->stream;
     k_fill_u64<<<1, 256, 0, s>>>(c->counters, n);
-    enqueue_sort(c);
+    enqueue_sort(c, take_order(c, false));
     enqueue_pair(c, true);
     if (c->L > 0) {
         k_ligcand<<<c->R, 64, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
@@ -933,7 +968,7 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
     double total = 0.0;
     if (rc == SDM_OK) {
         for (int s = 0; s < nsteps; s++) {
-            enqueue_sort(c);
+            enqueue_sort(c, take_order(c, false));
             (void)hipEventRecord(ev[2 * s], c->stream);
             enqueue_pair(c, false);
             (void)hipEventRecord(ev[2 * s + 1], c->stream);
@@ -970,9 +1005,12 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "cell_cells_per_dim out of range");
     if (t->pair_group != 0 && t->pair_group != 8 && t->pair_group != 16)
         return fail(SDM_ERR_INVALID, "pair_group must be 0, 8 or 16");
+    if (t->sort_interval < 1 || t->sort_interval > 100000)
+        return fail(SDM_ERR_INVALID, "sort_interval must be in [1, 100000]");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
+    c->since_sort = 0;
     drop_graph(c);
     return SDM_OK;
 }
@@ -1019,6 +1057,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
             }
             CHK(upload(c, c->x64, x.data(), n3));
             CHK(upload(c, c->v64, v.data(), n3));
+            c->since_sort = 0;
             CHK(upload(c, c->state, st.data(), c->R));
             CHK(upload(c, c->flags, fl.data(), c->R));
         }

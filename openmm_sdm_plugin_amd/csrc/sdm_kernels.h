@@ -94,7 +94,7 @@ __global__ void k_scatter(int N, int Np, int NC, const int* __restrict__ key,
     if (i >= N) return;
     int k = key[(size_t)r * Np + i];
     int s = cstart[(size_t)r * (NC + 1) + k] + atomicAdd(&cfill[(size_t)r * NC + k], 1);
-    orig_s[(size_t)r * Np + s] = i;
+    if (s >= 0 && s < N) orig_s[(size_t)r * Np + s] = i;  // always, with consistent counts
 }
 
 // per cell: order its slots by atom index, then gather the sorted FP32 arrays and clear
@@ -109,7 +109,7 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= NC) return;
     const int* st = cstart + (size_t)r * (NC + 1);
-    int beg = st[c], end = st[c + 1];
+    int beg = max(st[c], 0), end = min(st[c + 1], N);
     int* o = orig_s + (size_t)r * Np;
     for (int a = beg + 1; a < end; a++) {
         int v = o[a];
@@ -149,6 +149,55 @@ __global__ __launch_bounds__(64) void k_bbox(int N, int Np, int NB,
     int s = b * 64 + lane;
     bool v = s < N;
     float4 p = posq_s[(size_t)r * Np + s];
+    float lx = wave_min(v ? p.x : 3.0e38f), ly = wave_min(v ? p.y : 3.0e38f),
+          lz = wave_min(v ? p.z : 3.0e38f);
+    float hx = wave_max(v ? p.x : -3.0e38f), hy = wave_max(v ? p.y : -3.0e38f),
+          hz = wave_max(v ? p.z : -3.0e38f);
+    if (lane == 0) {
+        bb_lo[(size_t)r * NB + b] = make_float4(lx, ly, lz, 0.f);
+        bb_hi[(size_t)r * NB + b] = make_float4(hx, hy, hz, 0.f);
+    }
+}
+
+// Between full sorts the slot order of the last sort is kept.  One wave per 64-slot
+// block re-gathers its atoms' coordinates from the FP64 positions, each wrapped to the
+// periodic image nearest the block's first atom (itself wrapped into the box), so a
+// block stays compact when one of its atoms crosses a face of the box; then the block's
+// box is recomputed and the slot-ordered force accumulator cleared.  Culling and pair
+// distances take minimum images of box centres or of pairs, so they hold for any image.
+__global__ __launch_bounds__(64) void k_refresh(int N, int Np, int NB,
+                                                const double* __restrict__ x64,
+                                                const int* __restrict__ orig_s,
+                                                const float* __restrict__ qs, Geom g,
+                                                float4* __restrict__ posq_s,
+                                                float4* __restrict__ arec,
+                                                long long* __restrict__ F,
+                                                float4* __restrict__ bb_lo,
+                                                float4* __restrict__ bb_hi) {
+    int r = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
+    int sl = b * 64 + lane;
+    bool v = sl < N;  // slot b * 64 (lane 0) is always below N
+    const int* O = orig_s + (size_t)r * Np;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    int i = O[v ? sl : b * 64];
+    double wx = X[i], wy = X[Np + i], wz = X[2 * Np + i];
+    double rx = __shfl(wx, 0, 64), ry = __shfl(wy, 0, 64), rz = __shfl(wz, 0, 64);
+    rx -= g.Lx * floor(rx * g.iLx);
+    ry -= g.Ly * floor(ry * g.iLy);
+    rz -= g.Lz * floor(rz * g.iLz);
+    wx -= g.Lx * rint((wx - rx) * g.iLx);
+    wy -= g.Ly * rint((wy - ry) * g.iLy);
+    wz -= g.Lz * rint((wz - rz) * g.iLz);
+    float4 p = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
+    size_t o = (size_t)r * Np + sl;
+    if (v) {
+        posq_s[o] = p;
+        arec[2 * o] = p;
+        long long* Fr = F + (size_t)r * 3 * Np;
+        Fr[sl] = 0;
+        Fr[Np + sl] = 0;
+        Fr[2 * Np + sl] = 0;
+    }
     float lx = wave_min(v ? p.x : 3.0e38f), ly = wave_min(v ? p.y : 3.0e38f),
           lz = wave_min(v ? p.z : 3.0e38f);
     float hx = wave_max(v ? p.x : -3.0e38f), hy = wave_max(v ? p.y : -3.0e38f),

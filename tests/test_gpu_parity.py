@@ -272,3 +272,81 @@ def test_no_displacement(hip_device):
     ctx, e, F = gpu_components(s, displacement=np.zeros((s.n, 3)))
     assert e["u"] == 0.0
     assert np.array_equal(F[_native.FORCE_STATE2], F[_native.FORCE_STATE1])
+
+
+def _translated_run(s, kernel, interval, nsteps, drift=30.0):
+    """nsteps of deterministic dynamics (no friction, no noise) with every atom given an
+    extra velocity of `drift` nm/ps along x, so that ~0.03 nm per step of the system
+    crosses the x faces of the box: blocks of the kept slot order straddle a face."""
+    print(f"translated run {s.name} kernel={kernel} interval={interval} steps={nsteps}",
+          flush=True)
+    p = systems.params_config1()
+    p.friction, p.temperature = 0.0, 0.0
+    v = np.array(s.velocities, dtype=np.float64)
+    v[:, 0] += drift
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_tuning(**KERNELS[kernel], sort_interval=interval)
+    ctx.set_params(0, p)
+    ctx.set_positions(0, s.positions)
+    ctx.set_velocities(0, v)
+    ctx.step(nsteps)
+    return ctx, p
+
+
+@pytest.mark.parametrize("kernel", ["block_n3", "block_full", "grp16"])
+def test_components_between_sorts(hip_device, kernel):
+    """Energies and forces evaluated in a kept slot order (sorted 10 steps and ~0.3 nm
+    of translation earlier, atoms wrapped across the x faces since) match the oracle at
+    the current positions."""
+    s = systems.make_config1(frame=1)
+    ctx, p = _translated_run(s, kernel, interval=50, nsteps=10)
+    x = ctx.get_positions(0)
+    L = np.asarray(s.box)
+    crossed = np.any(np.floor(x / L) != np.floor(np.asarray(s.positions) / L), axis=1)
+    assert crossed.sum() > 50, int(crossed.sum())
+    ctx.compute_components()
+    e = ctx.energies()[0]
+    assert e["flags"] == 0
+    topo = O.Topology(s)
+    E1, E2, Eb, F1, F2, Fb = O.reference_eval(topo, x)
+    U0, u, F0d, dF, _, _ = O.sdm_direct(topo, x)
+    assert close(e["U0"], E1), (e["U0"], E1)
+    assert abs(e["U1"] - E2) <= E_TOL * max(abs(E1), abs(E2)), (e["U1"], E2)
+    assert close(e["u"], u), (e["u"], u)
+    assert rms_rel(ctx.forces(0, _native.FORCE_STATE1), F1) < F_TOL
+    assert rms_rel(ctx.forces(0, _native.FORCE_STATE2), F2) < F_TOL
+    assert rms_rel(ctx.forces(0, _native.FORCE_BONDED), Fb) < F_TOL
+
+
+def test_sort_interval_trajectory(hip_device):
+    """Sorting every step or every 7th step gives the same trajectory up to the FP32
+    summation order of the pair forces; a kept order is bit-reproducible."""
+    s = systems.make_config1(frame=2)
+    a, _ = _translated_run(s, "block_n3", interval=1, nsteps=20)
+    b, _ = _translated_run(s, "block_n3", interval=7, nsteps=20)
+    c, _ = _translated_run(s, "block_n3", interval=7, nsteps=20)
+    xa, xb, xc = a.get_positions(0), b.get_positions(0), c.get_positions(0)
+    assert np.max(np.abs(xa - xb)) < 1e-6
+    assert np.array_equal(xb, xc)
+    ea, eb, ec = a.energies()[0], b.energies()[0], c.energies()[0]
+    assert close(eb["pot_energy"], ea["pot_energy"]), (eb["pot_energy"], ea["pot_energy"])
+    assert eb["pot_energy"] == ec["pot_energy"] and eb["u"] == ec["u"]
+    assert ea["flags"] == eb["flags"] == 0
+
+
+def test_energy_calls_keep_step_order(hip_device):
+    """Energy evaluations between steps do not change the slot orders the steps use."""
+    s = systems.make_config1(frame=0)
+    a, _ = _translated_run(s, "block_n3", interval=4, nsteps=0)
+    b, _ = _translated_run(s, "block_n3", interval=4, nsteps=0)
+    a.step(9)
+    b.compute()
+    for _ in range(3):
+        b.step(3)
+        b.compute_components()
+    assert np.array_equal(a.get_positions(0), b.get_positions(0))
+    a.compute()
+    b.compute()
+    ea, eb = a.energies()[0], b.energies()[0]
+    assert ea["U0"] == eb["U0"] and ea["u"] == eb["u"] and ea["pot_energy"] == eb["pot_energy"]
+    assert np.array_equal(a.forces(0), b.forces(0))
