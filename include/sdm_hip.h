@@ -136,6 +136,12 @@ typedef struct {
                                      steps in between keep the slot order and only refresh
                                      the coordinates and block boxes.  Results differ only
                                      in the FP32 summation order of the pair energies. */
+    int32_t energy_every_step;    /* 0: sdm_step evaluates the pair energies (U0, U1,
+                                     pot_energy) on the last step of each call only, the
+                                     step whose record sdm_get_energies returns (u, W, sp,
+                                     the work and the forces are evaluated every step);
+                                     1: on every step.  Block Newton-3 kernel only; the
+                                     other pair kernels always evaluate them. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -210,9 +216,11 @@ int sdm_get_u_history(sdm_context* ctx, int32_t replica, double* out, int32_t ca
    non-excluded pairs with r < cutoff at x (p0) and of pairs with d_i != d_j and
    r < cutoff at x+d (pdelta). */
 int sdm_count_pairs(sdm_context* ctx, int64_t* p0, int64_t* pdelta /* [n_replicas] */);
-/* Average duration (ms) of the nonbonded pair kernel: nsteps launches back to back, each
-   after a spatial sort and with nothing running beside it, timed with HIP events on the
-   context's stream.  Positions do not advance; forces must be recomputed afterwards. */
+/* Average duration (ms) of the nonbonded pair kernel as most steps run it (forces only
+   unless energy_every_step is set): nsteps launches back to back, each after the slot
+   order's refresh and tile lists and with nothing running beside it, timed with HIP
+   events on the context's stream.  Positions do not advance; forces must be recomputed
+   afterwards. */
 int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 
 /* Launch-configuration search (project-written autotuner): times each candidate on

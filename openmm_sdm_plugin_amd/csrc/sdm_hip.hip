@@ -90,11 +90,12 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
-    sdm_tuning tune{1, 0, 1, 1, 0, 20};
+    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
-    hipGraph_t graph = nullptr, graph_keep = nullptr;  // step with / without a full sort
-    hipGraphExec_t gexec = nullptr, gexec_keep = nullptr;
+    // captured steps: [with a full sort][with the pair energies]
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t gexec[2][2] = {};
     bool components_valid = false;
     std::vector<void*> allocs;
 };
@@ -137,12 +138,13 @@ int download(sdm_context* c, T* dst, const T* src, size_t n) {
 }
 
 void drop_graph(sdm_context* c) {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    if (c->gexec_keep) (void)hipGraphExecDestroy(c->gexec_keep);
-    if (c->graph_keep) (void)hipGraphDestroy(c->graph_keep);
-    c->gexec = c->gexec_keep = nullptr;
-    c->graph = c->graph_keep = nullptr;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            if (c->gexec[a][b]) (void)hipGraphExecDestroy(c->gexec[a][b]);
+            if (c->graph[a][b]) (void)hipGraphDestroy(c->graph[a][b]);
+            c->gexec[a][b] = nullptr;
+            c->graph[a][b] = nullptr;
+        }
 }
 
 // Whether an evaluation sorts (true) or keeps the slot order of the last sort.  Steps
@@ -282,7 +284,8 @@ void launch_pair_grp_w(sdm_context* c, bool count, unsigned long long* pc) {
     }
 }
 
-void enqueue_pair(sdm_context* c, bool count) {
+// energy = false (block Newton-3 kernel only): forces without the pair energies
+void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
     hipStream_t s = c->stream;
     int waves = c->R * c->NB;
     unsigned long long* pc = count ? c->counters : nullptr;
@@ -297,8 +300,9 @@ void enqueue_pair(sdm_context* c, bool count) {
 #define SDM_LAUNCH_N3(W)                                                                      \
     do {                                                                                      \
         dim3 gr((waves + W - 1) / W);                                                         \
-        if (count) k_pair_n3<W, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                  \
-        else k_pair_n3<W, false><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                       \
+        if (count) k_pair_n3<W, true, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);            \
+        else if (energy) k_pair_n3<W, false, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);     \
+        else k_pair_n3<W, false, false><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                \
     } while (0)
         switch (c->tune.pair_waves_per_group) {
             case 2: SDM_LAUNCH_N3(2); break;
@@ -338,7 +342,7 @@ void enqueue_pair(sdm_context* c, bool count) {
 // work forks: the pair kernel on the main stream, the bonded terms and the displaced-state
 // chain on two side streams; they join before the scalar stage.  pe0/pe1 (optional) are
 // recorded around the pair kernel.
-void enqueue_eval(sdm_context* c, bool advance, bool components, bool full,
+void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
                   hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
     hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
@@ -369,14 +373,17 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full,
                                                   c->dB, c->g);
     }
     if (pe0) (void)hipEventRecord(pe0, s);
-    enqueue_pair(c, false);
+    // without the Newton-3 block kernel the energies are always evaluated
+    energy = energy || c->tune.pair_group != 0 || !c->tune.pair_newton3;
+    enqueue_pair(c, false, energy);
     if (pe1) (void)hipEventRecord(pe1, s);
     (void)hipEventRecord(c->ev_join[0], s1);
     (void)hipEventRecord(c->ev_join[1], s2);
     (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
     (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
-                               c->rec, c->flags, advance ? 1 : 0, c->hist_every, c->hist_cap,
+                               c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
+                               c->hist_every, c->hist_cap,
                                c->uhist);
     if (c->L > 0 && !components)
         k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
@@ -384,9 +391,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full,
                                                        c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
-void enqueue_step(sdm_context* c, bool full, hipEvent_t pe0 = nullptr,
+void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullptr,
                   hipEvent_t pe1 = nullptr) {
-    enqueue_eval(c, true, false, full, pe0, pe1);
+    enqueue_eval(c, true, false, full, energy, pe0, pe1);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
                                                            c->v64, c->rec, c->state, c->slot_of);
 }
@@ -397,11 +404,11 @@ int check_launch() {
     return SDM_OK;
 }
 
-int build_graph(sdm_context* c, bool full) {
-    hipGraph_t& graph = full ? c->graph : c->graph_keep;
-    hipGraphExec_t& gexec = full ? c->gexec : c->gexec_keep;
+int build_graph(sdm_context* c, bool full, bool energy) {
+    hipGraph_t& graph = c->graph[full][energy];
+    hipGraphExec_t& gexec = c->gexec[full][energy];
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, full);
+    enqueue_step(c, full, energy);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -789,7 +796,7 @@ int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
 
 int sdm_compute(sdm_context* c) {
     if (!c) return fail(SDM_ERR_INVALID, "null context");
-    enqueue_eval(c, false, false, take_order(c, false));
+    enqueue_eval(c, false, false, take_order(c, false), true);
     c->components_valid = false;
     return check_launch();
 }
@@ -801,11 +808,14 @@ int sdm_step(sdm_context* c, int32_t nsteps) {
     if (nsteps == 0) return SDM_OK;
     for (int s = 0; s < nsteps; s++) {
         bool full = take_order(c, true);
+        // the pair energies (U0, U1, pot_energy) on the last step of the call, the one
+        // whose record sdm_get_energies reads, or on every step if asked
+        bool energy = c->tune.energy_every_step || s == nsteps - 1;
         if (c->tune.use_graph) {
-            if (!(full ? c->gexec : c->gexec_keep)) CHK(build_graph(c, full));
-            HIPCHK(hipGraphLaunch(full ? c->gexec : c->gexec_keep, c->stream));
+            if (!c->gexec[full][energy]) CHK(build_graph(c, full, energy));
+            HIPCHK(hipGraphLaunch(c->gexec[full][energy], c->stream));
         } else {
-            enqueue_step(c, full);
+            enqueue_step(c, full, energy);
         }
     }
     return check_launch();
@@ -844,7 +854,7 @@ int sdm_compute_components(sdm_context* c) {
     }
     k_fill_ll<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->Fb_sep, n3);
     k_fill_d<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->dFd, n3);
-    enqueue_eval(c, false, true, take_order(c, false));
+    enqueue_eval(c, false, true, take_order(c, false), true);
     if (c->L > 0)
         k_scatter_dF<<<dim3(c->maxc + c->LB, c->R), 64, 0, c->stream>>>(
             c->N, c->Np, c->L, c->maxc, c->lig, c->cls, c->orig_s, c->cand, c->ncand, c->dB,
@@ -970,7 +980,7 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
         for (int s = 0; s < nsteps; s++) {
             enqueue_sort(c, take_order(c, false));
             (void)hipEventRecord(ev[2 * s], c->stream);
-            enqueue_pair(c, false);
+            enqueue_pair(c, false, c->tune.energy_every_step != 0);
             (void)hipEventRecord(ev[2 * s + 1], c->stream);
         }
         rc = check_launch();
@@ -1007,6 +1017,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_group must be 0, 8 or 16");
     if (t->sort_interval < 1 || t->sort_interval > 100000)
         return fail(SDM_ERR_INVALID, "sort_interval must be in [1, 100000]");
+    if (t->energy_every_step != 0 && t->energy_every_step != 1)
+        return fail(SDM_ERR_INVALID, "energy_every_step must be 0 or 1");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;

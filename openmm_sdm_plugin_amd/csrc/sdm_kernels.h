@@ -458,6 +458,8 @@ __device__ __forceinline__ float pair_one(float4 pi, float2 qi, float4 pj, float
 // mask outside the cutoff (a never-taken branch per pair would cost more and would keep
 // the scheduler from overlapping consecutive pairs).  Masked lanes may hold inf or NaN
 // before the mask (r = 0 for an unused slot); the AND yields +0.
+// ENERGY = false: forces only (the energy terms and their sums are left out)
+template <bool ENERGY = true>
 __device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, float2 qj,
                                              bool incl, bool minimg, const Geom& g, PairAcc& a,
                                              float& dx, float& dy, float& dz) {
@@ -479,20 +481,24 @@ __device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, fl
     float es6 = (qi.y * qj.y) * s6;
     float qq = pi.w * pj.w;
     float fr = es6 * (12.0f * s6 - 6.0f) * ir2 + qq * (ir * ir2 - 2.0f * g.krff);
-    float ee = es6 * (s6 - 1.0f) + qq * ir;
     fr = __uint_as_float(__float_as_uint(fr) & m);
-    ee = __uint_as_float(__float_as_uint(ee) & m);
-    float qqm = __uint_as_float(__float_as_uint(qq) & m);
     a.fx += fr * dx;
     a.fy += fr * dy;
     a.fz += fr * dz;
-    a.ef += ee;
-    a.sqr2 += qqm * r2;
-    a.sq += qqm;
     a.cnt += (int)(m & 1u);
     // pin the sums here: otherwise the compiler sinks every pair's accumulation to the
     // end of the chunk and keeps all eight pairs' operands live (VGPR pressure)
-    asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz), "+v"(a.ef), "+v"(a.sqr2), "+v"(a.sq));
+    if (ENERGY) {
+        float ee = es6 * (s6 - 1.0f) + qq * ir;
+        ee = __uint_as_float(__float_as_uint(ee) & m);
+        float qqm = __uint_as_float(__float_as_uint(qq) & m);
+        a.ef += ee;
+        a.sqr2 += qqm * r2;
+        a.sq += qqm;
+        asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz), "+v"(a.ef), "+v"(a.sqr2), "+v"(a.sq));
+    } else {
+        asm volatile("" : "+v"(a.fx), "+v"(a.fy), "+v"(a.fz));
+    }
     return fr;
 }
 
@@ -752,6 +758,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 //   vmcnt discipline: the only vector memory operations inside the tile loop are the
 //   next tile's copy and the three flush atomics that end each tile, so waiting for
 //   vmcnt <= 3 after the atomics means the copy has landed.
+//   ENERGY = false: forces only; e_pair is not written.
 // ---------------------------------------------------------------------------
 // s_waitcnt vmcnt(3) alone
 __device__ __forceinline__ void wait_vmem3() {
@@ -760,7 +767,7 @@ __device__ __forceinline__ void wait_vmem3() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int WPW, bool COUNT>
+template <int WPW, bool COUNT, bool ENERGY>
 __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
     int R, int N, int Np, int NB, int maxj, const float4* __restrict__ arec,
     const int* __restrict__ orig_s, const int* __restrict__ slot_of, const int* __restrict__ tiles,
@@ -870,15 +877,17 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                     bool incl = vi && jj[k] >= 0;
                     if (EX) incl = incl && !((em >> j) & 1ull);
                     float dx, dy, dz;
-                    float fr = pair_masked(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
+                    float fr = pair_masked<ENERGY>(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
                     v[3 * k] = -fr * dx;
                     v[3 * k + 1] = -fr * dy;
                     v[3 * k + 2] = -fr * dz;
                 }
                 float f[3];
                 reduce_scatter24(v, lane, f);
-                acc.e += (double)acc.ef;  // FP32 over 8 pairs, FP64 beyond
-                acc.ef = 0.f;
+                if (ENERGY) {
+                    acc.e += (double)acc.ef;  // FP32 over 8 pairs, FP64 beyond
+                    acc.ef = 0.f;
+                }
                 int kq = lane >> 3;
                 int mine = jj[0];
 #pragma unroll
@@ -924,8 +933,10 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
         ax += (double)acc.fx;
         ay += (double)acc.fy;
         az += (double)acc.fz;
-        aqr2 += (double)acc.sqr2;
-        aq += (double)acc.sq;
+        if (ENERGY) {
+            aqr2 += (double)acc.sqr2;
+            aq += (double)acc.sq;
+        }
         // flush the tile's j forces: lane = j atom, coalesced fixed-point atomics (the
         // sums are read and cleared the same way as they were added)
         float fjx, fjy, fjz;
@@ -976,9 +987,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
         atomicAdd(&Fr[Np + islot], (unsigned long long)fy);
         atomicAdd(&Fr[2 * Np + islot], (unsigned long long)fz);
     }
-    double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
-    if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
+    if (ENERGY) {
+        double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
+        if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1762,20 +1775,25 @@ __global__ __launch_bounds__(256) void k_scalar(
     int NB, int nbw, int LB, const double* __restrict__ e_pair,
     const double* __restrict__ e_bond, const double* __restrict__ u_part,
     DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
-    int advance, int hist_every, int hist_cap, double* __restrict__ uhist) {
+    int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist) {
     __shared__ double red[256];
     int r = blockIdx.x, t = threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
-    for (int k = t; k < NB; k += 256) s0 += e_pair[(size_t)r * NB + k];
-    for (int k = t; k < nbw; k += 256) s1 += e_bond[(size_t)r * nbw + k];
-    red[t] = s0;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) red[t] += red[t + o];
+    // energy == 0: the pair kernel left the pair energies out; U0, U1 and pot_energy
+    // keep the values of the last evaluation that had them (nothing else depends on U0)
+    double U0 = rec[r].U0;
+    if (energy) {
+        for (int k = t; k < NB; k += 256) s0 += e_pair[(size_t)r * NB + k];
+        red[t] = s0;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) red[t] += red[t + o];
+            __syncthreads();
+        }
+        U0 = red[0];
         __syncthreads();
     }
-    double U0 = red[0];
-    __syncthreads();
+    for (int k = t; k < nbw; k += 256) s1 += e_bond[(size_t)r * nbw + k];
     red[t] = s1;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -1840,10 +1858,14 @@ __global__ __launch_bounds__(256) void k_scalar(
     double sp = bfp * fp;
     double pot = U0 + ebias + Ub;
     int fl = 0;
-    if (!isfinite(U0) || !isfinite(u) || !isfinite(be) || !isfinite(pot) || !isfinite(sp)) fl |= 1;
+    if (!isfinite(u) || !isfinite(be) || !isfinite(sp)) fl |= 1;
+    if (energy && (!isfinite(U0) || !isfinite(pot))) fl |= 1;
     if (fl) atomicOr(&flags[r], fl);
-    Rr.U0 = U0;
-    Rr.U1 = U0 + u;
+    if (energy) {
+        Rr.U0 = U0;
+        Rr.U1 = U0 + u;
+        Rr.pot_energy = pot;
+    }
     Rr.u = u;
     Rr.Ub = Ub;
     Rr.bind_e = be;
@@ -1851,7 +1873,6 @@ __global__ __launch_bounds__(256) void k_scalar(
     Rr.W = ebias;
     Rr.Wp = bfp;
     Rr.sp = sp;
-    Rr.pot_energy = pot;
     Rr.lambda = lambdac;
     Rr.lambda1 = (method == 2) ? lambda1 : S.lambda1;
     Rr.lambda2 = (method == 2) ? lambda2 : S.lambda2;

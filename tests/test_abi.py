@@ -33,7 +33,7 @@ def test_struct_sizes_match_header_layout():
     # layout rule: every double is 8-aligned.
     assert C.sizeof(_native.sdm_params) == 8 + 8 * 11 + 8 + 8 * 13 + 8
     assert C.sizeof(_native.sdm_energies) == 8 * 17 + 8 + 8
-    assert C.sizeof(_native.sdm_tuning) == 24
+    assert C.sizeof(_native.sdm_tuning) == 28
 
 
 def _desc(s, **over):

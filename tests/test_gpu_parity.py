@@ -9,6 +9,7 @@ import pytest
 
 from openmm_sdm_plugin_amd import _native, systems
 from oracle import oracle as O
+from fp32_bounds import pair_abs_sum
 
 pytestmark = pytest.mark.gpu
 
@@ -30,31 +31,6 @@ def rms_rel(a, b):
 
 def close(a, b, tol=E_TOL, floor=1.0):
     return abs(a - b) <= tol * max(floor, abs(b))
-
-
-def pair_abs_sum(s, x):
-    """Sum of |E_LJ| + |E_Coulomb-RF| over the in-cutoff, non-excluded pairs (FP64, brute
-    force; small systems only).  FP32 pair arithmetic on exact FP32 positions has an
-    error of a few units of 2^-24 times this sum, whatever the (possibly strongly
-    cancelling) total: the 186-atom test system has U0 ~ 30 kJ/mol from a sum of
-    |pair energies| ~ 4.5e4, where an FP32 emulation of the kernel's arithmetic alone
-    is off by 3.5e-4, above 1e-5 relative."""
-    L = np.asarray(s.box)
-    d = x[:, None, :] - x[None, :, :]
-    d -= L * np.round(d / L)
-    i, j = np.triu_indices(s.n, 1)
-    r = np.sqrt((d[i, j] ** 2).sum(-1))
-    ex = set(map(tuple, np.sort(np.asarray(s.exclusions).reshape(-1, 2), axis=1).tolist()))
-    keep = (r < s.cutoff) & np.array([(a, b) not in ex for a, b in zip(i.tolist(), j.tolist())])
-    i, j, r = i[keep], j[keep], r[keep]
-    sig = 0.5 * (s.sigma[i] + s.sigma[j])
-    eps = np.sqrt(s.epsilon[i] * s.epsilon[j])
-    krf = (s.eps_rf - 1) / ((2 * s.eps_rf + 1) * s.cutoff ** 3)
-    crf = 3 * s.eps_rf / ((2 * s.eps_rf + 1) * s.cutoff)
-    sr6 = (sig / r) ** 6
-    lj = 4 * eps * sr6 * (sr6 - 1)
-    c = 138.935456 * s.charge[i] * s.charge[j] * (1 / r + krf * r * r - crf)
-    return float(np.sum(np.abs(lj) + np.abs(c)))
 
 
 def gpu_components(s, params=None, displacement=None, kernel="grp16"):
@@ -350,3 +326,31 @@ def test_energy_calls_keep_step_order(hip_device):
     ea, eb = a.energies()[0], b.energies()[0]
     assert ea["U0"] == eb["U0"] and ea["u"] == eb["u"] and ea["pot_energy"] == eb["pot_energy"]
     assert np.array_equal(a.forces(0), b.forces(0))
+
+
+def test_energy_on_last_step_of_call(hip_device):
+    """Pair energies are evaluated on the last step of each sdm_step call (the record
+    that energies() returns); the dynamics do not depend on them, so the trajectory and
+    the final energies equal those of a run that evaluates them every step."""
+    s = systems.make_config1(frame=1)
+    runs = []
+    for every in (0, 1):
+        ctx = _native.NativeContext(s, 1)
+        ctx.set_tuning(**KERNELS["block_n3"], energy_every_step=every)
+        p = systems.params_config1()
+        p.seed = 77
+        ctx.set_params(0, p)
+        ctx.set_positions(0, s.positions)
+        ctx.set_velocities(0, s.velocities)
+        ctx.step(7)
+        e7 = ctx.energies()[0]
+        ctx.step(1)
+        e8 = ctx.energies()[0]
+        runs.append((ctx.get_positions(0), e7, e8))
+    (xa, a7, a8), (xb, b7, b8) = runs
+    assert np.max(np.abs(xa - xb)) < 1e-9
+    for ea, eb in ((a7, b7), (a8, b8)):
+        for k in ("U0", "U1", "u", "pot_energy", "W", "sp"):
+            assert close(ea[k], eb[k], tol=1e-6), (k, ea[k], eb[k])
+        assert ea["flags"] == eb["flags"] == 0
+    assert a7["U0"] != a8["U0"]  # the single-step call evaluated its own energies
