@@ -1641,6 +1641,58 @@ __device__ __forceinline__ double delta_pair(const Atom64& I, const Atom64& J, c
     return ey - ex;
 }
 
+// Up to 64 atoms staged in LDS by one wave (lane t copies atom t), so that the loops over
+// partner atoms read LDS broadcasts instead of chains of dependent global loads.
+struct AtomTile64 {
+    double x[64], y[64], z[64], q[64], sig[64], eps[64];
+    int cls[64], idx[64];
+};
+
+__device__ __forceinline__ void stage_atom(AtomTile64& T, int t, const double* __restrict__ X,
+                                           int Np, int a, const int* __restrict__ cls,
+                                           const double* __restrict__ q,
+                                           const double* __restrict__ sig,
+                                           const double* __restrict__ eps) {
+    T.x[t] = X[a];
+    T.y[t] = X[Np + a];
+    T.z[t] = X[2 * Np + a];
+    T.q[t] = q[a];
+    T.sig[t] = sig[a];
+    T.eps[t] = eps[a];
+    T.cls[t] = cls[a];
+    T.idx[t] = a;
+}
+
+__device__ __forceinline__ Atom64 tile_atom(const AtomTile64& T, int t,
+                                            const double* __restrict__ cdispl) {
+    Atom64 A;
+    A.x = T.x[t];
+    A.y = T.y[t];
+    A.z = T.z[t];
+    A.cls = T.cls[t];
+    A.dx = cdispl[3 * A.cls];
+    A.dy = cdispl[3 * A.cls + 1];
+    A.dz = cdispl[3 * A.cls + 2];
+    A.qk = T.q[t];
+    A.sig = T.sig[t];
+    A.eps = T.eps[t];
+    A.idx = T.idx[t];
+    return A;
+}
+
+// whether atom a has an exclusion partner that is displaced (want_displaced) or not.
+// Exclusions across the boundary of the displaced set are rare (a bond from the ligand
+// to the receptor), so only atoms that have one search their list per pair.
+__device__ __forceinline__ bool excl_with(const int* __restrict__ ex_start,
+                                          const int* __restrict__ ex_list,
+                                          const int* __restrict__ cls, int a,
+                                          bool want_displaced) {
+    bool any = false;
+    for (int k = ex_start[a]; k < ex_start[a + 1]; k++)
+        any = any || ((cls[ex_list[k]] != 0) == want_displaced);
+    return any;
+}
+
 // lanes = displaced atoms (ligand block lb), j = undisplaced atoms of candidate block c.
 __global__ __launch_bounds__(64) void k_deltaA(
     int N, int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ cls,
@@ -1651,21 +1703,24 @@ __global__ __launch_bounds__(64) void k_deltaA(
     const int* __restrict__ ncand, int maxc, double4* __restrict__ dA, Geom g,
     unsigned long long* __restrict__ pair_count) {
     int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z, lane = threadIdx.x;
-    if (c >= ncand[r]) return;
+    if (c >= ncand[r]) return;  // uniform over the workgroup
+    __shared__ AtomTile64 T;
     const double* X = x64 + (size_t)r * 3 * Np;
     int li = lb * 64 + lane;
     bool v = li < L;
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
     int jb = cand[(size_t)r * maxc + c];
     int jbeg = jb * 64, jn = min(64, N - jbeg);
-    const int* O = orig_s + (size_t)r * Np;
+    if (lane < jn)
+        stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + jbeg + lane], cls, q64, sig64, eps64);
+    bool exI = v && excl_with(ex_start, ex_list, cls, I.idx, false);
+    __syncthreads();
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cnt = 0;
     for (int jj = 0; jj < jn; jj++) {
-        int b = O[jbeg + jj];
-        if (cls[b] != 0) continue;
-        Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
-        if (!v || excluded(ex_start, ex_list, I.idx, b)) continue;
+        if (T.cls[jj] != 0) continue;
+        Atom64 J = tile_atom(T, jj, cdispl);
+        if (!v || (exI && excluded(ex_start, ex_list, I.idx, J.idx))) continue;
         int iny;
         du += delta_pair(I, J, g, fx, fy, fz, &iny);
         cnt += iny;
@@ -1695,6 +1750,7 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int nc = ncand[r];
+#pragma unroll 8
     for (int c = 0; c < nc; c++) {
         double4 p = dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane];
         fx += p.x;
@@ -1702,14 +1758,20 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
         fz += p.z;
         du += p.w;
     }
+    __shared__ AtomTile64 T;
     int cnt = 0;
-    for (int k = 0; k < L; k++) {
-        int b = lig[k];
-        Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
-        if (!v || J.cls == I.cls || excluded(ex_start, ex_list, I.idx, b)) continue;
-        int iny;
-        du += 0.5 * delta_pair(I, J, g, fx, fy, fz, &iny);
-        cnt += (iny && I.idx < b) ? 1 : 0;
+    for (int base = 0; base < L; base += 64) {
+        int n = min(64, L - base);
+        __syncthreads();  // the previous chunk has been read
+        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, q64, sig64, eps64);
+        __syncthreads();
+        for (int k = 0; k < n; k++) {
+            Atom64 J = tile_atom(T, k, cdispl);
+            if (!v || J.cls == I.cls || excluded(ex_start, ex_list, I.idx, J.idx)) continue;
+            int iny;
+            du += 0.5 * delta_pair(I, J, g, fx, fy, fz, &iny);
+            cnt += (iny && I.idx < J.idx) ? 1 : 0;
+        }
     }
     if (pair_count) {
         double s = wave_sum((double)cnt);
@@ -1738,12 +1800,20 @@ __global__ __launch_bounds__(64) void k_deltaB(
     int b = v ? orig_s[(size_t)r * Np + s] : 0;
     Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
     bool env = v && J.cls == 0;
+    bool exJ = env && excl_with(ex_start, ex_list, cls, b, true);
+    __shared__ AtomTile64 T;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int k = 0; k < L; k++) {
-        Atom64 I = load_atom64(X, Np, lig[k], cls, cdispl, q64, sig64, eps64);
-        if (!env || excluded(ex_start, ex_list, b, I.idx)) continue;
-        int iny;
-        delta_pair(J, I, g, fx, fy, fz, &iny);
+    for (int base = 0; base < L; base += 64) {
+        int n = min(64, L - base);
+        __syncthreads();  // the previous chunk has been read
+        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, q64, sig64, eps64);
+        __syncthreads();
+        for (int k = 0; k < n; k++) {
+            Atom64 I = tile_atom(T, k, cdispl);
+            if (!env || (exJ && excluded(ex_start, ex_list, b, I.idx))) continue;
+            int iny;
+            delta_pair(J, I, g, fx, fy, fz, &iny);
+        }
     }
     dB[((size_t)r * maxc + c) * 64 + lane] = make_double4(fx, fy, fz, 0.0);
 }
