@@ -230,9 +230,9 @@ int setup_cells(sdm_context* c, int per_dim) {
 
 dim3 g2(int n, int per, int R) { return dim3((unsigned)((n + per - 1) / per), (unsigned)R); }
 
-// slot order (a full spatial sort, or the last sort's order refreshed), block boxes and
-// tile lists for every replica
-void enqueue_sort(sdm_context* c, bool full) {
+// slot order (a full spatial sort, or the last sort's order refreshed) and block boxes
+// for every replica
+void enqueue_order(sdm_context* c, bool full) {
     hipStream_t s = c->stream;
     int N = c->N, Np = c->Np, R = c->R;
     if (!full) {
@@ -248,10 +248,20 @@ void enqueue_sort(sdm_context* c, bool full) {
                                                 c->par_s, c->arec, c->F, c->g);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
     }
-    if (c->tune.pair_group == 0)  // the grouped kernel finds its j-blocks itself
-        k_tiles<<<dim3(c->NB, R), 256, 0, s>>>(N, Np, c->NB, c->maxj, c->posq_s, c->bb_lo,
-                                              c->bb_hi, c->tiles, c->tmask, c->ntiles,
-                                              c->list_r2, c->g, c->tune.pair_newton3, c->flags);
+}
+
+// tile lists of the block pair kernels (the grouped kernel finds its j-blocks itself)
+void enqueue_tiles(sdm_context* c) {
+    if (c->tune.pair_group == 0)
+        k_tiles<<<dim3(c->NB, c->R), 256, 0, c->stream>>>(
+            c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,
+            c->ntiles, c->list_r2, c->g, c->tune.pair_newton3, c->flags);
+}
+
+// slot order, block boxes and tile lists
+void enqueue_sort(sdm_context* c, bool full) {
+    enqueue_order(c, full);
+    enqueue_tiles(c);
 }
 
 // ring capacity (entries per group) of the grouped pair kernel
@@ -338,15 +348,18 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
 #undef SDM_LAUNCH_PAIR
 }
 
-// evaluation of one step (everything before the Langevin update).  After the sort the
-// work forks: the pair kernel on the main stream, the bonded terms and the displaced-state
-// chain on two side streams; they join before the scalar stage.  pe0/pe1 (optional) are
-// recorded around the pair kernel.
+// evaluation of one step (everything before the Langevin update).  After the slot order
+// and the tile lists the work forks: the pair kernel on the main stream, the bonded terms
+// and the displaced-state chain on two side streams; they join before the scalar stage.
+// (Forking before the tile lists, which the side work does not need, was measured
+// slower: the side kernels then slow the tile builder, which is on the critical path.)
+// pe0/pe1 (optional) are recorded around the pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
                   hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
     hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
-    enqueue_sort(c, full);
+    enqueue_order(c, full);
+    enqueue_tiles(c);
     (void)hipEventRecord(c->ev_fork, s);
     (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
     (void)hipStreamWaitEvent(s2, c->ev_fork, 0);

@@ -1680,6 +1680,20 @@ __device__ __forceinline__ Atom64 tile_atom(const AtomTile64& T, int t,
     return A;
 }
 
+// squared gap (FP64, minimum image of the centre difference) between point p and the box
+// [l, h]: a lower bound of the squared distance from p to every point of the box
+__device__ __forceinline__ double point_box_gap2(double px, double py, double pz, double lx,
+                                                 double ly, double lz, double hx, double hy,
+                                                 double hz, const Geom& g) {
+    double cx = min_image(px - 0.5 * (lx + hx), g.Lx, g.iLx);
+    double cy = min_image(py - 0.5 * (ly + hy), g.Ly, g.iLy);
+    double cz = min_image(pz - 0.5 * (lz + hz), g.Lz, g.iLz);
+    double gx = fmax(fabs(cx) - 0.5 * (hx - lx), 0.0);
+    double gy = fmax(fabs(cy) - 0.5 * (hy - ly), 0.0);
+    double gz = fmax(fabs(cz) - 0.5 * (hz - lz), 0.0);
+    return gx * gx + gy * gy + gz * gz;
+}
+
 // whether atom a has an exclusion partner that is displaced (want_displaced) or not.
 // Exclusions across the boundary of the displaced set are rare (a bond from the ligand
 // to the receptor), so only atoms that have one search their list per pair.
@@ -1714,12 +1728,26 @@ __global__ __launch_bounds__(64) void k_deltaA(
     if (lane < jn)
         stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + jbeg + lane], cls, q64, sig64, eps64);
     bool exI = v && excl_with(ex_start, ex_list, cls, I.idx, false);
+    // boxes of this block of displaced atoms at x and at x+d (lanes past L repeat atom 0
+    // of the ligand, which only widens them): j atoms beyond the cutoff of both boxes
+    // have no pair inside it at either geometry and are skipped (their terms are zero)
+    double b0lx = wave_min_d(I.x), b0ly = wave_min_d(I.y), b0lz = wave_min_d(I.z);
+    double b0hx = wave_max_d(I.x), b0hy = wave_max_d(I.y), b0hz = wave_max_d(I.z);
+    double b1lx = wave_min_d(I.x + I.dx), b1ly = wave_min_d(I.y + I.dy),
+           b1lz = wave_min_d(I.z + I.dz);
+    double b1hx = wave_max_d(I.x + I.dx), b1hy = wave_max_d(I.y + I.dy),
+           b1hz = wave_max_d(I.z + I.dz);
+    const double cut2 = g.rc2 * (1.0 + 1e-9);
     __syncthreads();
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cnt = 0;
     for (int jj = 0; jj < jn; jj++) {
         if (T.cls[jj] != 0) continue;
         Atom64 J = tile_atom(T, jj, cdispl);
+        if (point_box_gap2(J.x, J.y, J.z, b0lx, b0ly, b0lz, b0hx, b0hy, b0hz, g) >= cut2 &&
+            point_box_gap2(J.x + J.dx, J.y + J.dy, J.z + J.dz, b1lx, b1ly, b1lz, b1hx, b1hy,
+                           b1hz, g) >= cut2)
+            continue;  // wave-uniform
         if (!v || (exI && excluded(ex_start, ex_list, I.idx, J.idx))) continue;
         int iny;
         du += delta_pair(I, J, g, fx, fy, fz, &iny);
@@ -1801,15 +1829,31 @@ __global__ __launch_bounds__(64) void k_deltaB(
     Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
     bool env = v && J.cls == 0;
     bool exJ = env && excl_with(ex_start, ex_list, cls, b, true);
+    // box of the block's undisplaced atoms at x and at x+d (empty if it has none):
+    // displaced atoms beyond the cutoff of both have no pair inside it with any of them
+    double blx = wave_min_d(env ? J.x : 1e300), bly = wave_min_d(env ? J.y : 1e300),
+           blz = wave_min_d(env ? J.z : 1e300);
+    double bhx = wave_max_d(env ? J.x : -1e300), bhy = wave_max_d(env ? J.y : -1e300),
+           bhz = wave_max_d(env ? J.z : -1e300);
+    double dlx = wave_min_d(env ? J.x + J.dx : 1e300), dly = wave_min_d(env ? J.y + J.dy : 1e300),
+           dlz = wave_min_d(env ? J.z + J.dz : 1e300);
+    double dhx = wave_max_d(env ? J.x + J.dx : -1e300), dhy = wave_max_d(env ? J.y + J.dy : -1e300),
+           dhz = wave_max_d(env ? J.z + J.dz : -1e300);
+    const bool any_env = __ballot(env) != 0ull;
+    const double cut2 = g.rc2 * (1.0 + 1e-9);
     __shared__ AtomTile64 T;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int base = 0; base < L; base += 64) {
+    for (int base = 0; base < L && any_env; base += 64) {
         int n = min(64, L - base);
         __syncthreads();  // the previous chunk has been read
         if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, q64, sig64, eps64);
         __syncthreads();
         for (int k = 0; k < n; k++) {
             Atom64 I = tile_atom(T, k, cdispl);
+            if (point_box_gap2(I.x, I.y, I.z, blx, bly, blz, bhx, bhy, bhz, g) >= cut2 &&
+                point_box_gap2(I.x + I.dx, I.y + I.dy, I.z + I.dz, dlx, dly, dlz, dhx, dhy,
+                               dhz, g) >= cut2)
+                continue;  // wave-uniform
             if (!env || (exJ && excluded(ex_start, ex_list, b, I.idx))) continue;
             int iny;
             delta_pair(J, I, g, fx, fy, fz, &iny);
