@@ -169,6 +169,59 @@ int sdm_set_time(sdm_context* ctx, int32_t replica, double time, int64_t step);
 int sdm_set_nonbonded_params(sdm_context* ctx, const double* charge, const double* sigma,
                              const double* epsilon);
 
+/* Restraint forces of SDMUtils (python/SDMUtils.py).  Both kinds are force-group-1 terms:
+   evaluated at x (not at x+d), added to the bonded forces and to Ub
+   (LangevinIntegratorSDM.cpp:174-176).  Positions are taken as they are, without periodic
+   images, as the reference's CustomCentroidBondForce / CustomCompoundBondForce do.
+   Units: nm, rad, kJ/mol. */
+
+/* SDMUtils.addRestraintForce (SDMUtils.py:32-158), a CustomCentroidBondForce:
+     C*(kfcm/2)*step(d12-tolcm)*(d12-tolcm)^2,  d12 = |c1 - offset - c2|
+   with c1, c2 the weighted centres of the ligand and receptor groups, plus, with
+   do_angles, C*(kf/2)*(step(dm)*max(0,db)^2 + step(-dm)*max(0,-da)^2) for
+   theta = angle(g3,g6,g7) (period pi), phi1 = dihedral(g4,g3,g6,g7) and
+   phi2 = dihedral(g3,g6,g7,g8) (period 2 pi), where db, da, dm are theta - hi, theta - lo
+   and theta - (lo+hi)/2 wrapped to the period; g3..g5 = rcpt_ref, g6..g8 = lig_ref.
+   Dihedrals follow the IUPAC sign convention. */
+typedef struct {
+    int32_t n_lig, n_rcpt;
+    const int32_t* lig_atoms;  /* group g1 */
+    const int32_t* rcpt_atoms; /* group g2 */
+    const double* lig_weights; /* NULL: the particle masses (the force's default) */
+    const double* rcpt_weights;
+    double kfcm;               /* kJ/mol/nm^2 */
+    double tolcm;              /* nm */
+    double offset[3];          /* nm */
+    int32_t do_angles;         /* 1: the angle and dihedral terms */
+    int32_t rcpt_ref[3];
+    int32_t lig_ref[3];
+    int32_t pad;
+    double kf[3];              /* theta, phi1, phi2: kJ/mol/rad^2 */
+    double lo[3], hi[3];       /* flat-bottom windows: center -/+ tolerance, rad */
+    double control;            /* SDMRestraintControlParameter (1.0) */
+} sdm_centroid_restraint;
+
+/* SDMUtils.addAlignmentForce (SDMUtils.py:161-257): three CustomCompoundBondForces on the
+   reference atoms a0..a2 (liga_ref) and b0..b2 (ligb_ref):
+     (kfdispl/2)|b0 - offset - a0|^2
+   + (ktheta/2)(1 - cos t),  cos t = unit(b1-b0).unit(a1-a0)
+   + (kpsi/4)(1 - cos p(b0,b1,b2,a0,a2)) + (kpsi/4)(1 - cos p(a0,a1,a2,b0,b2))
+   where cos p(x1..x5) is the cosine between the components of x3-x1 and x5-x4
+   perpendicular to x2-x1. */
+typedef struct {
+    int32_t liga_ref[3];
+    int32_t ligb_ref[3];
+    double kfdispl;            /* kJ/mol/nm^2 */
+    double ktheta, kpsi;       /* kJ/mol */
+    double offset[3];          /* nm */
+} sdm_alignment_restraint;
+
+/* Replaces the context's restraints (n = 0 removes them).  No counterpart call in the
+   reference, where SDMUtils adds the forces to the System before the Context is built. */
+int sdm_set_restraints(sdm_context* ctx, const sdm_centroid_restraint* centroid,
+                       int32_t n_centroid, const sdm_alignment_restraint* alignment,
+                       int32_t n_alignment);
+
 /* State (host arrays, [3*n_atoms] doubles, atom order as given to sdm_create). */
 int sdm_set_positions(sdm_context* ctx, int32_t replica, const double* xyz);
 int sdm_get_positions(sdm_context* ctx, int32_t replica, double* xyz);

@@ -77,6 +77,22 @@ class sdm_tuning(C.Structure):
                 ("energy_every_step", C.c_int32)]
 
 
+class sdm_centroid_restraint(C.Structure):
+    _fields_ = [("n_lig", C.c_int32), ("n_rcpt", C.c_int32),
+                ("lig_atoms", C.POINTER(C.c_int32)), ("rcpt_atoms", C.POINTER(C.c_int32)),
+                ("lig_weights", C.POINTER(C.c_double)), ("rcpt_weights", C.POINTER(C.c_double)),
+                ("kfcm", C.c_double), ("tolcm", C.c_double), ("offset", C.c_double * 3),
+                ("do_angles", C.c_int32), ("rcpt_ref", C.c_int32 * 3),
+                ("lig_ref", C.c_int32 * 3), ("pad", C.c_int32), ("kf", C.c_double * 3),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("control", C.c_double)]
+
+
+class sdm_alignment_restraint(C.Structure):
+    _fields_ = [("liga_ref", C.c_int32 * 3), ("ligb_ref", C.c_int32 * 3),
+                ("kfdispl", C.c_double), ("ktheta", C.c_double), ("kpsi", C.c_double),
+                ("offset", C.c_double * 3)]
+
+
 # every exported symbol of include/sdm_hip.h
 EXPORTS = [
     "sdm_last_error", "sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params",
@@ -85,8 +101,8 @@ EXPORTS = [
     "sdm_get_positions_device", "sdm_compute", "sdm_step", "sdm_get_energies", "sdm_clear_flags",
     "sdm_compute_components", "sdm_get_forces", "sdm_kinetic_energy", "sdm_set_u_history",
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
-    "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "registerPlatforms",
-    "registerKernelFactories", "registerSDMHIPKernelFactories",
+    "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
+    "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
 _lib = None
@@ -129,6 +145,8 @@ def load(path: str = LIB_PATH):
     lib.sdm_get_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
     lib.sdm_set_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
     lib.sdm_du_dparams_frames.argtypes = [vp, vp, C.c_int32, _ip, C.c_int32, vp, vp]
+    lib.sdm_set_restraints.argtypes = [vp, C.POINTER(sdm_centroid_restraint), C.c_int32,
+                                       C.POINTER(sdm_alignment_restraint), C.c_int32]
     for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
               "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
               "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
@@ -136,7 +154,7 @@ def load(path: str = LIB_PATH):
               "sdm_get_energies", "sdm_clear_flags", "sdm_compute_components", "sdm_get_forces",
               "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
-              "sdm_du_dparams_frames"):
+              "sdm_du_dparams_frames", "sdm_set_restraints"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -219,6 +237,8 @@ class NativeContext:
         h = C.c_void_p()
         check(lib.sdm_create(C.byref(desc), n_replicas, C.byref(h)))
         self.h = h
+        if getattr(s, "restraints", None):
+            self.set_restraints(s.restraints)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -249,6 +269,42 @@ class NativeContext:
         s, sp = _d(sig)
         e, ep = _d(eps)
         check(self._lib.sdm_set_nonbonded_params(self.h, qp, sp, ep))
+
+    def set_restraints(self, restraints):
+        """Replaces the restraints (a list of restraints.CentroidRestraint and
+        restraints.AlignmentRestraint; [] removes them)."""
+        cen = [r for r in restraints if hasattr(r, "lig_atoms")]
+        ali = [r for r in restraints if hasattr(r, "liga_ref")]
+        keep = []
+        ca = (sdm_centroid_restraint * max(1, len(cen)))()
+        for k, r in enumerate(cen):
+            c = ca[k]
+            for name in ("lig_atoms", "rcpt_atoms"):
+                a = np.ascontiguousarray(r.__dict__[name], dtype=np.int32)
+                keep.append(a)
+                setattr(c, name, a.ctypes.data_as(C.POINTER(C.c_int32)))
+            c.n_lig, c.n_rcpt = len(r.lig_atoms), len(r.rcpt_atoms)
+            for name in ("lig_weights", "rcpt_weights"):
+                w = r.__dict__[name]
+                if w is not None:
+                    w = np.ascontiguousarray(w, dtype=np.float64)
+                    keep.append(w)
+                    setattr(c, name, w.ctypes.data_as(C.POINTER(C.c_double)))
+            c.kfcm, c.tolcm, c.control = float(r.kfcm), float(r.tolcm), float(r.control)
+            c.do_angles = int(r.do_angles)
+            for j in range(3):
+                c.offset[j] = float(r.offset[j])
+                c.rcpt_ref[j] = int(r.rcpt_ref[j])
+                c.lig_ref[j] = int(r.lig_ref[j])
+                c.kf[j], c.lo[j], c.hi[j] = float(r.kf[j]), float(r.lo[j]), float(r.hi[j])
+        aa = (sdm_alignment_restraint * max(1, len(ali)))()
+        for k, r in enumerate(ali):
+            a = aa[k]
+            for j in range(3):
+                a.liga_ref[j], a.ligb_ref[j] = int(r.liga_ref[j]), int(r.ligb_ref[j])
+                a.offset[j] = float(r.offset[j])
+            a.kfdispl, a.ktheta, a.kpsi = float(r.kfdispl), float(r.ktheta), float(r.kpsi)
+        check(self._lib.sdm_set_restraints(self.h, ca, len(cen), aa, len(ali)))
 
     def set_positions(self, replica, x):
         x, xp = _d(np.asarray(x).reshape(-1))

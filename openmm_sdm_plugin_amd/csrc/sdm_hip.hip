@@ -77,6 +77,14 @@ struct sdm_context {
     unsigned long long* tmask = nullptr;
     long long* F = nullptr;
     double *e_pair = nullptr, *e_bond = nullptr, *u_part = nullptr;
+    // SDMUtils restraints (group 1): centroid and alignment terms, their group atoms and
+    // normalized weights, and one energy slot per (replica, restraint)
+    DevCentroid* rcen = nullptr;
+    DevAlign* rali = nullptr;
+    int* ratom = nullptr;
+    double* rw = nullptr;
+    double* e_rest = nullptr;
+    int ncen = 0, nali = 0;
     int *cand = nullptr, *ncand = nullptr;
     double4 *dA = nullptr, *dlig = nullptr, *dB = nullptr;
     DevState* state = nullptr;
@@ -135,6 +143,14 @@ int download(sdm_context* c, T* dst, const T* src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SDM_OK;
+}
+
+// frees a buffer obtained from dalloc before the context ends
+void dfree(sdm_context* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
 }
 
 void drop_graph(sdm_context* c) {
@@ -369,6 +385,11 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
                                                   c->x64, components ? nullptr : c->slot_of,
                                                   components ? c->Fb_sep : c->F, c->e_bond,
                                                   c->nbw, c->flags);
+    if (c->ncen + c->nali > 0)
+        k_restraints<<<dim3(c->ncen + c->nali, R), 64, 0, s1>>>(
+            Np, c->ncen, c->ncen + c->nali, c->rcen, c->rali, c->ratom, c->rw, c->x64,
+            components ? nullptr : c->slot_of, components ? c->Fb_sep : c->F, c->e_rest,
+            c->flags);
     if (c->L > 0) {
         k_ligcand<<<R, 64, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
@@ -396,8 +417,8 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
-                               c->hist_every, c->hist_cap,
-                               c->uhist);
+                               c->hist_every, c->hist_cap, c->uhist, c->e_rest,
+                               c->ncen + c->nali);
     if (c->L > 0 && !components)
         k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
                                                        c->orig_s, c->cand, c->ncand, c->dB,
@@ -739,6 +760,103 @@ int sdm_set_nonbonded_params(sdm_context* c, const double* q, const double* sig,
     if (!c || !q || !sig || !eps) return fail(SDM_ERR_INVALID, "null argument");
     c->since_sort = 0;  // the slot-ordered parameter copies are rebuilt by the next sort
     return upload_nb_params(c, q, sig, eps);
+}
+
+int sdm_set_restraints(sdm_context* c, const sdm_centroid_restraint* cr, int32_t ncr,
+                       const sdm_alignment_restraint* ar, int32_t nar) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (ncr < 0 || nar < 0 || (ncr > 0 && !cr) || (nar > 0 && !ar))
+        return fail(SDM_ERR_INVALID, "bad restraint arrays");
+    auto atom_ok = [&](int a) { return a >= 0 && a < c->N; };
+    std::vector<double> mass(c->N);
+    CHK(download(c, mass.data(), c->mass, c->N));
+    std::vector<DevCentroid> hc(ncr);
+    std::vector<DevAlign> ha(nar);
+    std::vector<int> atoms;
+    std::vector<double> wts;
+    // one group: its atoms and weights normalized to sum 1 (masses by default)
+    auto add_group = [&](const int32_t* at, const double* w, int n, int* beg) {
+        if (n <= 0 || !at) return fail(SDM_ERR_INVALID, "empty restraint group");
+        double sum = 0.0;
+        for (int i = 0; i < n; i++) {
+            if (!atom_ok(at[i])) return fail(SDM_ERR_INVALID, "restraint atom out of range");
+            sum += w ? w[i] : mass[at[i]];
+        }
+        if (!(sum > 0.0) || !std::isfinite(sum))
+            return fail(SDM_ERR_INVALID, "restraint group weights must sum to > 0");
+        *beg = (int)atoms.size();
+        for (int i = 0; i < n; i++) {
+            atoms.push_back(at[i]);
+            wts.push_back((w ? w[i] : mass[at[i]]) / sum);
+        }
+        return SDM_OK;
+    };
+    for (int k = 0; k < ncr; k++) {
+        const sdm_centroid_restraint& q = cr[k];
+        DevCentroid& d = hc[k];
+        d = DevCentroid{};
+        CHK(add_group(q.lig_atoms, q.lig_weights, q.n_lig, &d.g1));
+        CHK(add_group(q.rcpt_atoms, q.rcpt_weights, q.n_rcpt, &d.g2));
+        d.n1 = q.n_lig;
+        d.n2 = q.n_rcpt;
+        d.do_angles = q.do_angles ? 1 : 0;
+        for (int j = 0; j < 3; j++) {
+            d.ref[j] = q.rcpt_ref[j];
+            d.ref[3 + j] = q.lig_ref[j];
+            d.off[j] = q.offset[j];
+            d.kf[j] = q.kf[j];
+            d.lo[j] = q.lo[j];
+            d.hi[j] = q.hi[j];
+        }
+        if (d.do_angles)
+            for (int j = 0; j < 6; j++)
+                if (!atom_ok(d.ref[j]))
+                    return fail(SDM_ERR_INVALID, "restraint reference atom out of range");
+        d.kfcm = q.kfcm;
+        d.tol = q.tolcm;
+        d.control = q.control;
+    }
+    for (int k = 0; k < nar; k++) {
+        const sdm_alignment_restraint& q = ar[k];
+        DevAlign& d = ha[k];
+        for (int j = 0; j < 3; j++) {
+            d.a[j] = q.liga_ref[j];
+            d.b[j] = q.ligb_ref[j];
+            d.off[j] = q.offset[j];
+            if (!atom_ok(d.a[j]) || !atom_ok(d.b[j]))
+                return fail(SDM_ERR_INVALID, "alignment reference atom out of range");
+        }
+        d.kfd = q.kfdispl;
+        d.kth = q.ktheta;
+        d.kps = q.kpsi;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, c->rcen);
+    dfree(c, c->rali);
+    dfree(c, c->ratom);
+    dfree(c, c->rw);
+    dfree(c, c->e_rest);
+    c->rcen = nullptr;
+    c->rali = nullptr;
+    c->ratom = nullptr;
+    c->rw = nullptr;
+    c->e_rest = nullptr;
+    c->ncen = c->nali = 0;
+    drop_graph(c);
+    if (ncr + nar == 0) return SDM_OK;
+    CHK(dalloc(c, &c->rcen, hc.size()));
+    CHK(dalloc(c, &c->rali, ha.size()));
+    CHK(dalloc(c, &c->ratom, atoms.size()));
+    CHK(dalloc(c, &c->rw, wts.size()));
+    CHK(dalloc(c, &c->e_rest, (size_t)c->R * (ncr + nar)));
+    CHK(upload(c, c->rcen, hc.data(), hc.size()));
+    CHK(upload(c, c->rali, ha.data(), ha.size()));
+    CHK(upload(c, c->ratom, atoms.data(), atoms.size()));
+    CHK(upload(c, c->rw, wts.data(), wts.size()));
+    c->ncen = ncr;
+    c->nali = nar;
+    c->components_valid = false;
+    return SDM_OK;
 }
 
 static int set_vec(sdm_context* c, double* base, int32_t r, const double* xyz) {

@@ -1522,6 +1522,289 @@ __global__ __launch_bounds__(256) void k_bonded(
 }
 
 // ---------------------------------------------------------------------------
+// group 1: the SDMUtils restraints at x (FP64), fixed-point atomics; one wave per
+// (restraint, replica).  Restates SDMUtils.py:32-257 (the energy expressions of the
+// CustomCentroidBondForce and the three CustomCompoundBondForces); no periodic images.
+// ---------------------------------------------------------------------------
+struct DevCentroid {
+    int g1, n1, g2, n2;  // ranges of the restraint atom / weight arrays (weights sum to 1)
+    int do_angles, pad;
+    int ref[6];          // g3, g4, g5 (receptor), g6, g7, g8 (ligand)
+    double kfcm, tol, off[3], control;
+    double kf[3], lo[3], hi[3];
+};
+
+struct DevAlign {
+    int a[3], b[3];
+    double kfd, kth, kps, off[3];
+};
+
+__device__ __forceinline__ double wrap_period(double x, double P) {
+    return x - P * floor(x / P + 0.5);
+}
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// (kf/2)(step(dm) max(0,db)^2 + step(-dm) max(0,-da)^2) with db, da, dm = t - hi, t - lo,
+// t - (lo+hi)/2 wrapped to the period P (step(0) = 1); *dEdt = its derivative in t
+__device__ double flat_bottom(double t, double kf, double lo, double hi, double P,
+                              double* dEdt) {
+    double db = wrap_period(t - hi, P), da = wrap_period(t - lo, P);
+    double dm = wrap_period(t - 0.5 * (lo + hi), P);
+    double e = 0.0, g = 0.0;
+    if (dm >= 0.0) {
+        double m = fmax(0.0, db);
+        e += m * m;
+        g += m;
+    }
+    if (-dm >= 0.0) {
+        double m = fmax(0.0, -da);
+        e += m * m;
+        g -= m;
+    }
+    *dEdt = kf * g;
+    return 0.5 * kf * e;
+}
+
+// angle at p2 in [0, pi] and its gradient (zero when the three points are collinear)
+__device__ double angle_grad(const double* p1, const double* p2, const double* p3, double* g1,
+                             double* g2, double* g3) {
+    double u[3], v[3], n[3], nu[3], nv[3];
+    for (int d = 0; d < 3; d++) {
+        u[d] = p1[d] - p2[d];
+        v[d] = p3[d] - p2[d];
+    }
+    cross3(u, v, n);
+    double nn = sqrt(dot3(n, n));
+    double t = atan2(nn, dot3(u, v));
+    cross3(n, u, nu);
+    cross3(n, v, nv);
+    double su = nn > 0.0 ? 1.0 / (nn * dot3(u, u)) : 0.0;
+    double sv = nn > 0.0 ? 1.0 / (nn * dot3(v, v)) : 0.0;
+    for (int d = 0; d < 3; d++) {
+        g1[d] = -nu[d] * su;
+        g3[d] = nv[d] * sv;
+        g2[d] = -(g1[d] + g3[d]);
+    }
+    return t;
+}
+
+// dihedral (IUPAC sign) in (-pi, pi] and its gradient (zero when degenerate)
+__device__ double dihedral_grad(const double* p1, const double* p2, const double* p3,
+                                const double* p4, double* g1, double* g2, double* g3,
+                                double* g4) {
+    double b1[3], b2[3], b3[3], n1[3], n2[3];
+    for (int d = 0; d < 3; d++) {
+        b1[d] = p2[d] - p1[d];
+        b2[d] = p3[d] - p2[d];
+        b3[d] = p4[d] - p3[d];
+    }
+    cross3(b1, b2, n1);
+    cross3(b2, b3, n2);
+    double l2 = sqrt(dot3(b2, b2));
+    double t = atan2(l2 * dot3(b1, n2), dot3(n1, n2));
+    double q1 = dot3(n1, n1), q2 = dot3(n2, n2);
+    bool ok = q1 > 0.0 && q2 > 0.0 && l2 > 0.0;
+    double s1 = ok ? dot3(b1, b2) / (l2 * l2) : 0.0, s3 = ok ? dot3(b3, b2) / (l2 * l2) : 0.0;
+    for (int d = 0; d < 3; d++) {
+        g1[d] = ok ? -l2 * n1[d] / q1 : 0.0;
+        g4[d] = ok ? l2 * n2[d] / q2 : 0.0;
+        g2[d] = (-s1 - 1.0) * g1[d] + s3 * g4[d];
+        g3[d] = s1 * g1[d] - (s3 + 1.0) * g4[d];
+    }
+    return t;
+}
+
+// k (1 - cos p) for cos p between the components of x3-x1 and x5-x4 perpendicular to
+// x2-x1; adds the forces to f[0..4]
+__device__ double psi_term(const double* x1, const double* x2, const double* x3,
+                           const double* x4, const double* x5, double k, double (*f)[3]) {
+    double d1[3], d0[3], d3[3], n1[3], v[3], w[3];
+    for (int d = 0; d < 3; d++) {
+        d1[d] = x2[d] - x1[d];
+        d0[d] = x3[d] - x1[d];
+        d3[d] = x5[d] - x4[d];
+    }
+    double l1 = sqrt(dot3(d1, d1));
+    for (int d = 0; d < 3; d++) n1[d] = d1[d] / l1;
+    double p0 = dot3(d0, n1), p3 = dot3(d3, n1);
+    for (int d = 0; d < 3; d++) {
+        v[d] = d0[d] - p0 * n1[d];
+        w[d] = d3[d] - p3 * n1[d];
+    }
+    double lv = sqrt(dot3(v, v)), lw = sqrt(dot3(w, w));
+    double c = dot3(v, w) / (lv * lw);
+    for (int d = 0; d < 3; d++) {
+        double gv = (w[d] / lw - c * v[d] / lv) / lv;
+        double gw = (v[d] / lv - c * w[d] / lw) / lw;
+        double gd1 = (-p0 * gv - p3 * gw) / l1;
+        // F = -dE/dx = k d(cos p)/dx
+        f[0][d] -= k * (gd1 + gv);
+        f[1][d] += k * gd1;
+        f[2][d] += k * gv;
+        f[3][d] -= k * gw;
+        f[4][d] += k * gw;
+    }
+    return k * (1.0 - c);
+}
+
+__global__ __launch_bounds__(64) void k_restraints(
+    int Np, int ncen, int nrest, const DevCentroid* __restrict__ cen,
+    const DevAlign* __restrict__ ali, const int* __restrict__ ratom,
+    const double* __restrict__ rw, const double* __restrict__ x64,
+    const int* __restrict__ slot_of, long long* __restrict__ F, double* __restrict__ e_rest,
+    int* __restrict__ flags) {
+    const double kPi = 3.14159265358979323846;
+    int k = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    const int* S = slot_of ? slot_of + (size_t)r * Np : nullptr;
+    int bad = 0;
+    auto addf = [&](int a, const double* f) {
+        int s = S ? S[a] : a;
+        atomicAdd(&Fr[s], (unsigned long long)to_fixed(f[0], &bad));
+        atomicAdd(&Fr[Np + s], (unsigned long long)to_fixed(f[1], &bad));
+        atomicAdd(&Fr[2 * Np + s], (unsigned long long)to_fixed(f[2], &bad));
+    };
+    auto pos = [&](int a, double* p) {
+        p[0] = X[a];
+        p[1] = X[Np + a];
+        p[2] = X[2 * Np + a];
+    };
+    double e = 0.0;
+    if (k < ncen) {
+        const DevCentroid& R = cen[k];
+        // weighted centres of the two groups (wave butterflies: every lane gets the sums)
+        double c[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = lane; i < R.n1; i += 64) {
+            int a = ratom[R.g1 + i];
+            double w = rw[R.g1 + i];
+            c[0] += w * X[a];
+            c[1] += w * X[Np + a];
+            c[2] += w * X[2 * Np + a];
+        }
+        for (int i = lane; i < R.n2; i += 64) {
+            int a = ratom[R.g2 + i];
+            double w = rw[R.g2 + i];
+            c[3] += w * X[a];
+            c[4] += w * X[Np + a];
+            c[5] += w * X[2 * Np + a];
+        }
+        for (int d = 0; d < 6; d++) c[d] = wave_sum(c[d]);
+        double dv[3] = {c[0] - R.off[0] - c[3], c[1] - R.off[1] - c[4],
+                        c[2] - R.off[2] - c[5]};
+        double d12 = sqrt(dot3(dv, dv));
+        double gd = 0.0;
+        if (d12 >= R.tol && d12 > 0.0) {
+            double x = d12 - R.tol;
+            e += 0.5 * R.control * R.kfcm * x * x;
+            gd = R.control * R.kfcm * x / d12;  // |dE/dc1| / d12
+        }
+        if (gd != 0.0) {  // the same on every lane
+            for (int i = lane; i < R.n1; i += 64) {
+                double w = rw[R.g1 + i];
+                double f[3] = {-w * gd * dv[0], -w * gd * dv[1], -w * gd * dv[2]};
+                addf(ratom[R.g1 + i], f);
+            }
+            for (int i = lane; i < R.n2; i += 64) {
+                double w = rw[R.g2 + i];
+                double f[3] = {w * gd * dv[0], w * gd * dv[1], w * gd * dv[2]};
+                addf(ratom[R.g2 + i], f);
+            }
+        }
+        if (R.do_angles && lane == 0) {
+            double p[6][3], f[6][3] = {}, g[4][3], dE;
+            for (int j = 0; j < 6; j++) pos(R.ref[j], p[j]);
+            // theta = angle(g3, g6, g7)
+            double t = angle_grad(p[0], p[3], p[4], g[0], g[1], g[2]);
+            e += R.control * flat_bottom(t, R.kf[0], R.lo[0], R.hi[0], kPi, &dE);
+            dE *= R.control;
+            const int ta[3] = {0, 3, 4};
+            for (int j = 0; j < 3; j++)
+                for (int d = 0; d < 3; d++) f[ta[j]][d] -= dE * g[j][d];
+            // phi1 = dihedral(g4, g3, g6, g7), phi2 = dihedral(g3, g6, g7, g8)
+            const int da[2][4] = {{1, 0, 3, 4}, {0, 3, 4, 5}};
+            for (int q = 0; q < 2; q++) {
+                const int* ix = da[q];
+                t = dihedral_grad(p[ix[0]], p[ix[1]], p[ix[2]], p[ix[3]], g[0], g[1], g[2],
+                                  g[3]);
+                e += R.control *
+                     flat_bottom(t, R.kf[1 + q], R.lo[1 + q], R.hi[1 + q], 2.0 * kPi, &dE);
+                dE *= R.control;
+                for (int j = 0; j < 4; j++)
+                    for (int d = 0; d < 3; d++) f[ix[j]][d] -= dE * g[j][d];
+            }
+            for (int j = 0; j < 6; j++) addf(R.ref[j], f[j]);
+        }
+    } else if (lane == 0) {
+        const DevAlign& A = ali[k - ncen];
+        double a[3][3], b[3][3], fa[3][3] = {}, fb[3][3] = {};
+        for (int j = 0; j < 3; j++) {
+            pos(A.a[j], a[j]);
+            pos(A.b[j], b[j]);
+        }
+        // displacement: (kfdispl/2)|b0 - offset - a0|^2
+        double dd[3];
+        for (int d = 0; d < 3; d++) dd[d] = b[0][d] - A.off[d] - a[0][d];
+        e += 0.5 * A.kfd * dot3(dd, dd);
+        for (int d = 0; d < 3; d++) {
+            fb[0][d] -= A.kfd * dd[d];
+            fa[0][d] += A.kfd * dd[d];
+        }
+        // theta: (ktheta/2)(1 - cos t), cos t = unit(b1 - b0).unit(a1 - a0)
+        double d1[3], d2[3];
+        for (int d = 0; d < 3; d++) {
+            d1[d] = b[1][d] - b[0][d];
+            d2[d] = a[1][d] - a[0][d];
+        }
+        double l1 = sqrt(dot3(d1, d1)), l2 = sqrt(dot3(d2, d2));
+        double cs = dot3(d1, d2) / (l1 * l2);
+        e += 0.5 * A.kth * (1.0 - cs);
+        for (int d = 0; d < 3; d++) {
+            double f2 = 0.5 * A.kth * (d2[d] / l2 - cs * d1[d] / l1) / l1;
+            double f4 = 0.5 * A.kth * (d1[d] / l1 - cs * d2[d] / l2) / l2;
+            fb[1][d] += f2;
+            fb[0][d] -= f2;
+            fa[1][d] += f4;
+            fa[0][d] -= f4;
+        }
+        // psi, symmetrized: (kpsi/4)(1 - cos p) for (b0,b1,b2,a0,a2) and (a0,a1,a2,b0,b2)
+        double f5[5][3] = {};
+        e += psi_term(b[0], b[1], b[2], a[0], a[2], 0.25 * A.kps, f5);
+        for (int d = 0; d < 3; d++) {
+            fb[0][d] += f5[0][d];
+            fb[1][d] += f5[1][d];
+            fb[2][d] += f5[2][d];
+            fa[0][d] += f5[3][d];
+            fa[2][d] += f5[4][d];
+        }
+        double h5[5][3] = {};
+        e += psi_term(a[0], a[1], a[2], b[0], b[2], 0.25 * A.kps, h5);
+        for (int d = 0; d < 3; d++) {
+            fa[0][d] += h5[0][d];
+            fa[1][d] += h5[1][d];
+            fa[2][d] += h5[2][d];
+            fb[0][d] += h5[3][d];
+            fb[2][d] += h5[4][d];
+        }
+        for (int j = 0; j < 3; j++) {
+            addf(A.a[j], fa[j]);
+            addf(A.b[j], fb[j]);
+        }
+    }
+    if (bad) atomicOr(&flags[r], 2);
+    if (lane == 0) e_rest[(size_t)r * nrest + k] = e;
+}
+
+// ---------------------------------------------------------------------------
 // displaced-state passes (FP64)
 // ---------------------------------------------------------------------------
 
@@ -1889,7 +2172,8 @@ __global__ __launch_bounds__(256) void k_scalar(
     int NB, int nbw, int LB, const double* __restrict__ e_pair,
     const double* __restrict__ e_bond, const double* __restrict__ u_part,
     DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
-    int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist) {
+    int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist,
+    const double* __restrict__ e_rest, int nrest) {
     __shared__ double red[256];
     int r = blockIdx.x, t = threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
@@ -1916,6 +2200,7 @@ __global__ __launch_bounds__(256) void k_scalar(
     }
     double Ub = red[0];
     if (t != 0) return;
+    for (int k = 0; k < nrest; k++) Ub += e_rest[(size_t)r * nrest + k];  // restraints
     double u = 0.0;
     for (int k = 0; k < LB; k++) u += u_part[(size_t)r * LB + k];
     DevState& S = state[r];

@@ -104,6 +104,8 @@ class SDMSystem:
     periodic: bool = True
     ligand: Optional[np.ndarray] = None  # displaced atom indices
     name: str = ""
+    # group-1 restraints (restraints.CentroidRestraint / AlignmentRestraint)
+    restraints: list = dataclasses.field(default_factory=list)
 
     @property
     def n(self) -> int:
