@@ -47,6 +47,7 @@ extern "C" {
 #define SDM_FLAG_NONFINITE 1  /* an energy or u' was not finite */
 #define SDM_FLAG_OVERFLOW 2   /* a force exceeded the 2^31 kJ/mol/nm fixed-point range */
 #define SDM_FLAG_CAPACITY 4   /* a neighbour list was truncated */
+#define SDM_FLAG_CONSTRAINT 8 /* a constraint cluster did not converge */
 
 /* System description: replaces the OpenMM System + NonbondedForce (group 2) +
    HarmonicBond/AngleForce (group 1) + the integrator's displacement map
@@ -215,6 +216,55 @@ typedef struct {
     double ktheta, kpsi;       /* kJ/mol */
     double offset[3];          /* nm */
 } sdm_alignment_restraint;
+
+/* Further force-field terms of a Desmond DMS system (example/desmonddmsfile75.py:546-850).
+   Group 1 (at x, into the bonded forces and Ub):
+     periodic torsions   k (1 + cos(n phi - phase))        (PeriodicTorsionForce)
+     harmonic impropers  k (phi - phi0)^2                   (CustomTorsionForce, :684-717)
+     flat-bottom posre   k (|x - x0| - tol)^2 beyond tol, minimum image (:972-1017)
+     harmonic posre      sum_d k_d (x_d - x0_d)^2           (:917-970)
+   (phi with the IUPAC sign).  Group 2: the 1-4 exceptions of the NonbondedForce, replacing
+   the pair (which must also be an exclusion): 4 eps ((s/r)^12 - (s/r)^6) + k_e qq / r with
+   no cutoff, no reaction field and no periodic image (:812-845; OpenMM's exceptions).  Both
+   atoms of an exception must have the same displacement (their energy is then the same at
+   x and x+d); it is added to U0 and U1 and its force to both states.  Constraints: fixed
+   distances kept by the Langevin update (positions after each step, velocities as the
+   constrained displacement over dt), as OpenMM's LangevinIntegrator does; solved by SHAKE
+   to relative tolerance constraint_tol (the connected atoms of a cluster share one
+   thread).  Units nm, rad, kJ/mol, e. */
+typedef struct {
+    int32_t n_torsions;
+    const int32_t* torsion_atoms;   /* [4*n] */
+    const int32_t* torsion_n;       /* periodicity */
+    const double* torsion_phase;    /* rad */
+    const double* torsion_k;        /* kJ/mol */
+    int32_t n_impropers;
+    const int32_t* improper_atoms;  /* [4*n] */
+    const double* improper_phi0;    /* rad */
+    const double* improper_k;       /* kJ/mol/rad^2 */
+    int32_t n_posre_fb;
+    const int32_t* posre_fb_atoms;
+    const double* posre_fb_x0;      /* [3*n] nm */
+    const double* posre_fb_k;       /* kJ/mol/nm^2 */
+    const double* posre_fb_tol;     /* nm */
+    int32_t n_posre_harm;
+    const int32_t* posre_harm_atoms;
+    const double* posre_harm_x0;    /* [3*n] nm */
+    const double* posre_harm_k;     /* [3*n] kJ/mol/nm^2 per axis */
+    int32_t n_exceptions;
+    const int32_t* exception_atoms; /* [2*n] */
+    const double* exception_qq;     /* e^2 */
+    const double* exception_sigma;  /* nm */
+    const double* exception_epsilon;/* kJ/mol */
+    int32_t n_constraints;
+    const int32_t* constraint_atoms;/* [2*n] */
+    const double* constraint_length;/* nm */
+    double constraint_tol;          /* relative; 0 = 1e-10 */
+} sdm_forcefield_terms;
+
+/* Replaces the context's further force-field terms (NULL removes them).  No counterpart
+   call in the reference, whose DMS reader adds these forces to the System. */
+int sdm_set_forcefield_terms(sdm_context* ctx, const sdm_forcefield_terms* terms);
 
 /* Replaces the context's restraints (n = 0 removes them).  No counterpart call in the
    reference, where SDMUtils adds the forces to the System before the Context is built. */

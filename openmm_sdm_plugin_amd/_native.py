@@ -93,6 +93,24 @@ class sdm_alignment_restraint(C.Structure):
                 ("offset", C.c_double * 3)]
 
 
+_I, _D = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+
+
+class sdm_forcefield_terms(C.Structure):
+    _fields_ = [("n_torsions", C.c_int32), ("torsion_atoms", _I), ("torsion_n", _I),
+                ("torsion_phase", _D), ("torsion_k", _D),
+                ("n_impropers", C.c_int32), ("improper_atoms", _I), ("improper_phi0", _D),
+                ("improper_k", _D),
+                ("n_posre_fb", C.c_int32), ("posre_fb_atoms", _I), ("posre_fb_x0", _D),
+                ("posre_fb_k", _D), ("posre_fb_tol", _D),
+                ("n_posre_harm", C.c_int32), ("posre_harm_atoms", _I), ("posre_harm_x0", _D),
+                ("posre_harm_k", _D),
+                ("n_exceptions", C.c_int32), ("exception_atoms", _I), ("exception_qq", _D),
+                ("exception_sigma", _D), ("exception_epsilon", _D),
+                ("n_constraints", C.c_int32), ("constraint_atoms", _I),
+                ("constraint_length", _D), ("constraint_tol", C.c_double)]
+
+
 # every exported symbol of include/sdm_hip.h
 EXPORTS = [
     "sdm_last_error", "sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params",
@@ -102,6 +120,7 @@ EXPORTS = [
     "sdm_compute_components", "sdm_get_forces", "sdm_kinetic_energy", "sdm_set_u_history",
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
     "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
+    "sdm_set_forcefield_terms",
     "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
@@ -147,6 +166,7 @@ def load(path: str = LIB_PATH):
     lib.sdm_du_dparams_frames.argtypes = [vp, vp, C.c_int32, _ip, C.c_int32, vp, vp]
     lib.sdm_set_restraints.argtypes = [vp, C.POINTER(sdm_centroid_restraint), C.c_int32,
                                        C.POINTER(sdm_alignment_restraint), C.c_int32]
+    lib.sdm_set_forcefield_terms.argtypes = [vp, C.POINTER(sdm_forcefield_terms)]
     for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
               "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
               "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
@@ -154,7 +174,7 @@ def load(path: str = LIB_PATH):
               "sdm_get_energies", "sdm_clear_flags", "sdm_compute_components", "sdm_get_forces",
               "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
-              "sdm_du_dparams_frames", "sdm_set_restraints"):
+              "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -239,6 +259,8 @@ class NativeContext:
         self.h = h
         if getattr(s, "restraints", None):
             self.set_restraints(s.restraints)
+        if getattr(s, "terms", None) is not None:
+            self.set_forcefield_terms(s.terms)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -305,6 +327,44 @@ class NativeContext:
                 a.offset[j] = float(r.offset[j])
             a.kfdispl, a.ktheta, a.kpsi = float(r.kfdispl), float(r.ktheta), float(r.kpsi)
         check(self._lib.sdm_set_restraints(self.h, ca, len(cen), aa, len(ali)))
+
+    def set_forcefield_terms(self, ff, constraint_tol=0.0):
+        """Torsions, impropers, positional restraints, 1-4 exceptions and constraints of a
+        dms.DMSForceField (or any object with its array attributes); None removes them."""
+        if ff is None:
+            check(self._lib.sdm_set_forcefield_terms(self.h, None))
+            return
+        t = sdm_forcefield_terms()
+        keep = []
+
+        def put(field, arr, dtype, count_field=None, rows=None):
+            a = np.ascontiguousarray(arr, dtype=dtype)
+            keep.append(a)
+            setattr(t, field, a.ctypes.data_as(_I if dtype == np.int32 else _D))
+            if count_field:
+                setattr(t, count_field, int(a.shape[0] if rows is None else rows))
+        put("torsion_atoms", ff.torsions, np.int32, "n_torsions")
+        put("torsion_n", ff.torsion_n, np.int32)
+        put("torsion_phase", ff.torsion_phase, np.float64)
+        put("torsion_k", ff.torsion_k, np.float64)
+        put("improper_atoms", ff.impropers, np.int32, "n_impropers")
+        put("improper_phi0", ff.improper_t0, np.float64)
+        put("improper_k", ff.improper_k, np.float64)
+        put("posre_fb_atoms", ff.posre_fb_atoms, np.int32, "n_posre_fb")
+        put("posre_fb_x0", ff.posre_fb_x0, np.float64)
+        put("posre_fb_k", ff.posre_fb_hk, np.float64)
+        put("posre_fb_tol", ff.posre_fb_tol, np.float64)
+        put("posre_harm_atoms", ff.posre_h_atoms, np.int32, "n_posre_harm")
+        put("posre_harm_x0", ff.posre_h_x0, np.float64)
+        put("posre_harm_k", ff.posre_h_hk, np.float64)
+        put("exception_atoms", ff.exception_pairs, np.int32, "n_exceptions")
+        put("exception_qq", ff.exception_qq, np.float64)
+        put("exception_sigma", ff.exception_sigma, np.float64)
+        put("exception_epsilon", ff.exception_epsilon, np.float64)
+        put("constraint_atoms", ff.constraints, np.int32, "n_constraints")
+        put("constraint_length", ff.constraint_r, np.float64)
+        t.constraint_tol = float(constraint_tol)
+        check(self._lib.sdm_set_forcefield_terms(self.h, C.byref(t)))
 
     def set_positions(self, replica, x):
         x, xp = _d(np.asarray(x).reshape(-1))

@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sdm_kernels.h"
@@ -85,6 +86,19 @@ struct sdm_context {
     double* rw = nullptr;
     double* e_rest = nullptr;
     int ncen = 0, nali = 0;
+    // further force-field terms (torsions, impropers, positional restraints, 1-4
+    // exceptions): device arrays behind `terms`, per-workgroup energies of groups 1 and 2
+    DevTerms terms{};
+    int nterm = 0, ntb = 0;
+    std::vector<void*> term_bufs;
+    double *e_t1 = nullptr, *e_t2 = nullptr;
+    // constraints: clusters (atoms, and constraints as local index pairs with squared
+    // lengths), the per-atom flag and the displacement scratch of the Langevin update
+    int ncl = 0;
+    int *cl_start = nullptr, *cl_atoms = nullptr, *con_start = nullptr, *con_ij = nullptr;
+    int* cflag = nullptr;
+    double *con_d2 = nullptr, *dxs = nullptr;
+    double con_tol = 1e-10;
     int *cand = nullptr, *ncand = nullptr;
     double4 *dA = nullptr, *dlig = nullptr, *dB = nullptr;
     DevState* state = nullptr;
@@ -390,6 +404,11 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             Np, c->ncen, c->ncen + c->nali, c->rcen, c->rali, c->ratom, c->rw, c->x64,
             components ? nullptr : c->slot_of, components ? c->Fb_sep : c->F, c->e_rest,
             c->flags);
+    if (c->nterm > 0)
+        k_terms<<<dim3(c->ntb, R), 256, 0, s1>>>(
+            Np, c->terms, c->x64, components ? nullptr : c->slot_of,
+            components ? c->Fb_sep : c->F, c->slot_of, c->F, c->e_t1, c->e_t2, c->ntb, c->g,
+            c->flags);
     if (c->L > 0) {
         k_ligcand<<<R, 64, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
@@ -418,7 +437,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
-                               c->ncen + c->nali);
+                               c->ncen + c->nali, c->e_t1, c->e_t2, c->ntb);
     if (c->L > 0 && !components)
         k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
                                                        c->orig_s, c->cand, c->ncand, c->dB,
@@ -429,7 +448,12 @@ void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullp
                   hipEvent_t pe1 = nullptr) {
     enqueue_eval(c, true, false, full, energy, pe0, pe1);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
-                                                           c->v64, c->rec, c->state, c->slot_of);
+                                                           c->v64, c->rec, c->state, c->slot_of,
+                                                           c->cflag, c->dxs);
+    if (c->ncl > 0)
+        k_constrain<<<g2(c->ncl, 256, c->R), 256, 0, c->stream>>>(
+            c->Np, c->ncl, c->cl_start, c->cl_atoms, c->con_start, c->con_ij, c->con_d2,
+            c->con_tol, c->invm, c->dxs, c->x64, c->v64, c->rec, c->flags);
 }
 
 int check_launch() {
@@ -760,6 +784,189 @@ int sdm_set_nonbonded_params(sdm_context* c, const double* q, const double* sig,
     if (!c || !q || !sig || !eps) return fail(SDM_ERR_INVALID, "null argument");
     c->since_sort = 0;  // the slot-ordered parameter copies are rebuilt by the next sort
     return upload_nb_params(c, q, sig, eps);
+}
+
+// constraint clusters (connected components of the constraint graph) for k_constrain;
+// t == nullptr removes them
+static int set_constraints(sdm_context* c, const sdm_forcefield_terms* t) {
+    int nc = t ? t->n_constraints : 0;
+    std::vector<int> parent(c->N);
+    for (int i = 0; i < c->N; i++) parent[i] = i;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    std::vector<double> invm(c->N);
+    if (nc > 0) CHK(download(c, invm.data(), c->invm, c->N));
+    for (int k = 0; k < nc; k++) {
+        int a = t->constraint_atoms[2 * k], b = t->constraint_atoms[2 * k + 1];
+        if (a == b || !(t->constraint_length[k] > 0.0))
+            return fail(SDM_ERR_INVALID, "a constraint needs two atoms and a positive length");
+        if (invm[a] == 0.0 || invm[b] == 0.0)
+            return fail(SDM_ERR_INVALID, "constrained atoms must have mass");
+        parent[find(a)] = find(b);
+    }
+    // clusters in order of their first atom; atoms and constraints grouped per cluster
+    std::vector<int> root_cl(c->N, -1), local(c->N, -1), cl_start{0}, cl_atoms, flag(c->N, 0);
+    std::vector<std::vector<int>> members;
+    for (int k = 0; k < nc; k++)
+        for (int j = 0; j < 2; j++) flag[t->constraint_atoms[2 * k + j]] = 1;
+    for (int i = 0; i < c->N; i++) {
+        if (!flag[i]) continue;
+        int rt = find(i);
+        if (root_cl[rt] < 0) {
+            root_cl[rt] = (int)members.size();
+            members.emplace_back();
+        }
+        local[i] = (int)members[root_cl[rt]].size();
+        members[root_cl[rt]].push_back(i);
+    }
+    std::vector<std::vector<int>> cons(members.size());
+    for (int k = 0; k < nc; k++) cons[root_cl[find(t->constraint_atoms[2 * k])]].push_back(k);
+    std::vector<int> con_start{0}, con_ij;
+    std::vector<double> con_d2;
+    for (size_t q = 0; q < members.size(); q++) {
+        if ((int)members[q].size() > SHAKE_MAX_ATOMS || (int)cons[q].size() > SHAKE_MAX_CONS)
+            return fail(SDM_ERR_UNSUPPORTED, "a constraint cluster has more than 16 atoms or "
+                                             "32 constraints");
+        for (int a : members[q]) cl_atoms.push_back(a);
+        cl_start.push_back((int)cl_atoms.size());
+        for (int k : cons[q]) {
+            con_ij.push_back(local[t->constraint_atoms[2 * k]]);
+            con_ij.push_back(local[t->constraint_atoms[2 * k + 1]]);
+            con_d2.push_back(t->constraint_length[k] * t->constraint_length[k]);
+        }
+        con_start.push_back((int)con_d2.size());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void* p : {(void*)c->cl_start, (void*)c->cl_atoms, (void*)c->con_start,
+                    (void*)c->con_ij, (void*)c->con_d2, (void*)c->cflag, (void*)c->dxs})
+        dfree(c, p);
+    c->cl_start = c->cl_atoms = c->con_start = c->con_ij = c->cflag = nullptr;
+    c->con_d2 = c->dxs = nullptr;
+    c->ncl = 0;
+    drop_graph(c);
+    if (nc == 0) return SDM_OK;
+    c->con_tol = t->constraint_tol > 0.0 ? t->constraint_tol : 1e-10;
+    CHK(dalloc(c, &c->cl_start, cl_start.size()));
+    CHK(dalloc(c, &c->cl_atoms, cl_atoms.size()));
+    CHK(dalloc(c, &c->con_start, con_start.size()));
+    CHK(dalloc(c, &c->con_ij, con_ij.size()));
+    CHK(dalloc(c, &c->con_d2, con_d2.size()));
+    CHK(dalloc(c, &c->cflag, c->N));
+    CHK(dalloc(c, &c->dxs, 3 * (size_t)c->R * c->Np));
+    CHK(upload(c, c->cl_start, cl_start.data(), cl_start.size()));
+    CHK(upload(c, c->cl_atoms, cl_atoms.data(), cl_atoms.size()));
+    CHK(upload(c, c->con_start, con_start.data(), con_start.size()));
+    CHK(upload(c, c->con_ij, con_ij.data(), con_ij.size()));
+    CHK(upload(c, c->con_d2, con_d2.data(), con_d2.size()));
+    CHK(upload(c, c->cflag, flag.data(), c->N));
+    c->ncl = (int)members.size();
+    return SDM_OK;
+}
+
+int sdm_set_forcefield_terms(sdm_context* c, const sdm_forcefield_terms* t) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    sdm_forcefield_terms none{};
+    if (!t) t = &none;
+    if (t->n_torsions < 0 || t->n_impropers < 0 || t->n_posre_fb < 0 || t->n_posre_harm < 0 ||
+        t->n_exceptions < 0 || t->n_constraints < 0)
+        return fail(SDM_ERR_INVALID, "negative term count");
+    auto ok_atoms = [&](const int32_t* a, int n, int per) {
+        if (n > 0 && !a) return false;
+        for (int k = 0; k < n * per; k++)
+            if (a[k] < 0 || a[k] >= c->N) return false;
+        return true;
+    };
+    auto ok_arr = [&](const void* p, int n) { return n == 0 || p != nullptr; };
+    if (!ok_atoms(t->torsion_atoms, t->n_torsions, 4) ||
+        !ok_atoms(t->improper_atoms, t->n_impropers, 4) ||
+        !ok_atoms(t->posre_fb_atoms, t->n_posre_fb, 1) ||
+        !ok_atoms(t->posre_harm_atoms, t->n_posre_harm, 1) ||
+        !ok_atoms(t->exception_atoms, t->n_exceptions, 2) ||
+        !ok_atoms(t->constraint_atoms, t->n_constraints, 2))
+        return fail(SDM_ERR_INVALID, "term atom index out of range");
+    if (!ok_arr(t->torsion_n, t->n_torsions) || !ok_arr(t->torsion_phase, t->n_torsions) ||
+        !ok_arr(t->torsion_k, t->n_torsions) || !ok_arr(t->improper_phi0, t->n_impropers) ||
+        !ok_arr(t->improper_k, t->n_impropers) || !ok_arr(t->posre_fb_x0, t->n_posre_fb) ||
+        !ok_arr(t->posre_fb_k, t->n_posre_fb) || !ok_arr(t->posre_fb_tol, t->n_posre_fb) ||
+        !ok_arr(t->posre_harm_x0, t->n_posre_harm) || !ok_arr(t->posre_harm_k, t->n_posre_harm) ||
+        !ok_arr(t->exception_qq, t->n_exceptions) || !ok_arr(t->exception_sigma, t->n_exceptions) ||
+        !ok_arr(t->exception_epsilon, t->n_exceptions) ||
+        !ok_arr(t->constraint_length, t->n_constraints))
+        return fail(SDM_ERR_INVALID, "null term array");
+    if (t->n_exceptions > 0) {  // both atoms of an exception move together under d
+        std::vector<int> cls(c->N);
+        CHK(download(c, cls.data(), c->cls, c->N));
+        for (int k = 0; k < t->n_exceptions; k++)
+            if (cls[t->exception_atoms[2 * k]] != cls[t->exception_atoms[2 * k + 1]])
+                return fail(SDM_ERR_UNSUPPORTED,
+                            "an exception pair's atoms must have the same displacement");
+    }
+    CHK(set_constraints(c, t->n_constraints > 0 ? t : nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void* p : c->term_bufs) dfree(c, p);
+    c->term_bufs.clear();
+    dfree(c, c->e_t1);
+    dfree(c, c->e_t2);
+    c->e_t1 = c->e_t2 = nullptr;
+    c->terms = DevTerms{};
+    c->nterm = c->ntb = 0;
+    drop_graph(c);
+    c->components_valid = false;
+    int nterm = t->n_torsions + t->n_impropers + t->n_posre_fb + t->n_posre_harm + t->n_exceptions;
+    if (nterm == 0) return SDM_OK;
+    // device copies; the exception charge products include k_e
+    auto put = [&](auto** dst, const auto* src, size_t n) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        T* p = nullptr;
+        int rc = dalloc(c, &p, n);
+        if (rc == SDM_OK) rc = upload(c, p, src, n);
+        if (rc == SDM_OK) c->term_bufs.push_back(p);
+        *dst = p;
+        return rc;
+    };
+    DevTerms& T = c->terms;
+    T.nt = t->n_torsions;
+    T.ni = t->n_impropers;
+    T.nfb = t->n_posre_fb;
+    T.nph = t->n_posre_harm;
+    T.nex = t->n_exceptions;
+    if (T.nt) {
+        CHK(put(&T.t_atoms, t->torsion_atoms, 4 * (size_t)T.nt));
+        CHK(put(&T.t_n, t->torsion_n, T.nt));
+        CHK(put(&T.t_phase, t->torsion_phase, T.nt));
+        CHK(put(&T.t_k, t->torsion_k, T.nt));
+    }
+    if (T.ni) {
+        CHK(put(&T.i_atoms, t->improper_atoms, 4 * (size_t)T.ni));
+        CHK(put(&T.i_phi0, t->improper_phi0, T.ni));
+        CHK(put(&T.i_k, t->improper_k, T.ni));
+    }
+    if (T.nfb) {
+        CHK(put(&T.fb_atom, t->posre_fb_atoms, T.nfb));
+        CHK(put(&T.fb_x0, t->posre_fb_x0, 3 * (size_t)T.nfb));
+        CHK(put(&T.fb_k, t->posre_fb_k, T.nfb));
+        CHK(put(&T.fb_tol, t->posre_fb_tol, T.nfb));
+    }
+    if (T.nph) {
+        CHK(put(&T.ph_atom, t->posre_harm_atoms, T.nph));
+        CHK(put(&T.ph_x0, t->posre_harm_x0, 3 * (size_t)T.nph));
+        CHK(put(&T.ph_k, t->posre_harm_k, 3 * (size_t)T.nph));
+    }
+    if (T.nex) {
+        std::vector<double> qq(T.nex);
+        for (int k = 0; k < T.nex; k++) qq[k] = SDM_ONE_4PI_EPS0 * t->exception_qq[k];
+        CHK(put(&T.ex_atoms, t->exception_atoms, 2 * (size_t)T.nex));
+        CHK(put(&T.ex_qq, qq.data(), T.nex));
+        CHK(put(&T.ex_sig, t->exception_sigma, T.nex));
+        CHK(put(&T.ex_eps, t->exception_epsilon, T.nex));
+    }
+    c->nterm = nterm;
+    c->ntb = (nterm + 255) / 256;
+    CHK(dalloc(c, &c->e_t1, (size_t)c->R * c->ntb));
+    CHK(dalloc(c, &c->e_t2, (size_t)c->R * c->ntb));
+    return SDM_OK;
 }
 
 int sdm_set_restraints(sdm_context* c, const sdm_centroid_restraint* cr, int32_t ncr,

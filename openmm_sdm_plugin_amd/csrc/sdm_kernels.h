@@ -1805,6 +1805,134 @@ __global__ __launch_bounds__(64) void k_restraints(
 }
 
 // ---------------------------------------------------------------------------
+// further force-field terms (FP64, one thread per term): periodic torsions, harmonic
+// impropers and positional restraints (group 1), the 1-4 exceptions of the nonbonded
+// force (group 2: plain LJ + Coulomb, no cutoff, no periodic image).
+// ---------------------------------------------------------------------------
+struct DevTerms {
+    int nt, ni, nfb, nph, nex;  // term index ranges, in this order
+    const int* t_atoms;
+    const int* t_n;
+    const double *t_phase, *t_k;
+    const int* i_atoms;
+    const double *i_phi0, *i_k;
+    const int* fb_atom;
+    const double *fb_x0, *fb_k, *fb_tol;
+    const int* ph_atom;
+    const double *ph_x0, *ph_k;
+    const int* ex_atoms;
+    const double *ex_qq, *ex_sig, *ex_eps;  // qq includes k_e
+};
+
+// group-1 forces go to Fb (slot map Sb, or atom order if null), group-2 forces to F
+// (slot map S); energies per workgroup into e1 (group 1) and e2 (group 2)
+__global__ __launch_bounds__(256) void k_terms(int Np, DevTerms T, const double* __restrict__ x64,
+                                               const int* __restrict__ Sb_all,
+                                               long long* __restrict__ Fb,
+                                               const int* __restrict__ S_all,
+                                               long long* __restrict__ F, double* __restrict__ e1,
+                                               double* __restrict__ e2, int nblk, Geom g,
+                                               int* __restrict__ flags) {
+    __shared__ double red[2][256];
+    int r = blockIdx.y, t = blockIdx.x * 256 + threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    const int* Sb = Sb_all ? Sb_all + (size_t)r * Np : nullptr;
+    const int* S = S_all + (size_t)r * Np;
+    unsigned long long* Fbr = (unsigned long long*)(Fb + (size_t)r * 3 * Np);
+    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
+    int bad = 0;
+    double eg1 = 0.0, eg2 = 0.0;
+    auto pos = [&](int a, double* p) {
+        p[0] = X[a];
+        p[1] = X[Np + a];
+        p[2] = X[2 * Np + a];
+    };
+    auto addf = [&](unsigned long long* Fx, const int* map, int a, const double* f) {
+        int s = map ? map[a] : a;
+        atomicAdd(&Fx[s], (unsigned long long)to_fixed(f[0], &bad));
+        atomicAdd(&Fx[Np + s], (unsigned long long)to_fixed(f[1], &bad));
+        atomicAdd(&Fx[2 * Np + s], (unsigned long long)to_fixed(f[2], &bad));
+    };
+    const int e_imp = T.nt + T.ni, e_fb = e_imp + T.nfb, e_ph = e_fb + T.nph, e_ex = e_ph + T.nex;
+    if (t < e_imp) {  // torsion or improper: a function of the dihedral
+        bool tors = t < T.nt;
+        int k = tors ? t : t - T.nt;
+        const int* a = tors ? T.t_atoms + 4 * k : T.i_atoms + 4 * k;
+        double p[4][3], gr[4][3];
+        for (int j = 0; j < 4; j++) pos(a[j], p[j]);
+        double phi = dihedral_grad(p[0], p[1], p[2], p[3], gr[0], gr[1], gr[2], gr[3]);
+        double dE;
+        if (tors) {
+            double n = (double)T.t_n[k], arg = n * phi - T.t_phase[k];
+            eg1 = T.t_k[k] * (1.0 + cos(arg));
+            dE = -T.t_k[k] * n * sin(arg);
+        } else {
+            double d = phi - T.i_phi0[k];
+            eg1 = T.i_k[k] * d * d;
+            dE = 2.0 * T.i_k[k] * d;
+        }
+        for (int j = 0; j < 4; j++) {
+            double f[3] = {-dE * gr[j][0], -dE * gr[j][1], -dE * gr[j][2]};
+            addf(Fbr, Sb, a[j], f);
+        }
+    } else if (t < e_fb) {  // flat-bottom positional restraint, minimum image
+        int k = t - e_imp, a = T.fb_atom[k];
+        double p[3];
+        pos(a, p);
+        double d[3] = {min_image(p[0] - T.fb_x0[3 * k], g.Lx, g.iLx),
+                       min_image(p[1] - T.fb_x0[3 * k + 1], g.Ly, g.iLy),
+                       min_image(p[2] - T.fb_x0[3 * k + 2], g.Lz, g.iLz)};
+        double dist = sqrt(dot3(d, d)), x = dist - T.fb_tol[k];
+        if (x >= 0.0 && dist > 0.0) {
+            eg1 = T.fb_k[k] * x * x;
+            double c = -2.0 * T.fb_k[k] * x / dist;
+            double f[3] = {c * d[0], c * d[1], c * d[2]};
+            addf(Fbr, Sb, a, f);
+        }
+    } else if (t < e_ph) {  // harmonic positional restraint per axis
+        int k = t - e_fb, a = T.ph_atom[k];
+        double p[3], f[3];
+        pos(a, p);
+        for (int d = 0; d < 3; d++) {
+            double dx = p[d] - T.ph_x0[3 * k + d];
+            eg1 += T.ph_k[3 * k + d] * dx * dx;
+            f[d] = -2.0 * T.ph_k[3 * k + d] * dx;
+        }
+        addf(Fbr, Sb, a, f);
+    } else if (t < e_ex) {  // 1-4 exception (group 2)
+        int k = t - e_ph, i = T.ex_atoms[2 * k], j = T.ex_atoms[2 * k + 1];
+        double pi[3], pj[3], d[3];
+        pos(i, pi);
+        pos(j, pj);
+        for (int q = 0; q < 3; q++) d[q] = pi[q] - pj[q];
+        double r2 = dot3(d, d), ir2 = 1.0 / r2, ir = sqrt(ir2);
+        double s2 = T.ex_sig[k] * T.ex_sig[k] * ir2, s6 = s2 * s2 * s2;
+        double e4 = 4.0 * T.ex_eps[k];
+        eg2 = e4 * s6 * (s6 - 1.0) + T.ex_qq[k] * ir;
+        double fr = (e4 * s6 * (12.0 * s6 - 6.0) + T.ex_qq[k] * ir) * ir2;  // -dE/dr / r
+        double fi[3] = {fr * d[0], fr * d[1], fr * d[2]};
+        double fj[3] = {-fi[0], -fi[1], -fi[2]};
+        addf(Fr, S, i, fi);
+        addf(Fr, S, j, fj);
+    }
+    if (bad) atomicOr(&flags[r], 2);
+    red[0][threadIdx.x] = eg1;
+    red[1][threadIdx.x] = eg2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        e1[(size_t)r * nblk + blockIdx.x] = red[0][0];
+        e2[(size_t)r * nblk + blockIdx.x] = red[1][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // displaced-state passes (FP64)
 // ---------------------------------------------------------------------------
 
@@ -2173,7 +2301,8 @@ __global__ __launch_bounds__(256) void k_scalar(
     const double* __restrict__ e_bond, const double* __restrict__ u_part,
     DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
     int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist,
-    const double* __restrict__ e_rest, int nrest) {
+    const double* __restrict__ e_rest, int nrest, const double* __restrict__ e_t1,
+    const double* __restrict__ e_t2, int ntb) {
     __shared__ double red[256];
     int r = blockIdx.x, t = threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
@@ -2201,6 +2330,10 @@ __global__ __launch_bounds__(256) void k_scalar(
     double Ub = red[0];
     if (t != 0) return;
     for (int k = 0; k < nrest; k++) Ub += e_rest[(size_t)r * nrest + k];  // restraints
+    for (int k = 0; k < ntb; k++) {  // further force-field terms: groups 1 and 2
+        Ub += e_t1[(size_t)r * ntb + k];
+        if (energy) U0 += e_t2[(size_t)r * ntb + k];
+    }
     double u = 0.0;
     for (int k = 0; k < LB; k++) u += u_part[(size_t)r * LB + k];
     DevState& S = state[r];
@@ -2337,13 +2470,16 @@ __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
     if (bad) atomicOr(&flags[r], 2);
 }
 
-// leapfrog Langevin (langevin.cl:7-69, double-precision branch), no constraints.
+// leapfrog Langevin (langevin.cl:7-69, double-precision branch).  Atoms with a
+// constraint (cflag) leave their displacement dt*v in dxs for k_constrain.
 __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* __restrict__ invm,
                                                   const long long* __restrict__ F,
                                                   double* __restrict__ x64, double* __restrict__ v64,
                                                   const DevRecord* __restrict__ rec,
                                                   const DevState* __restrict__ state,
-                                                  const int* __restrict__ slot_of) {
+                                                  const int* __restrict__ slot_of,
+                                                  const int* __restrict__ cflag,
+                                                  double* __restrict__ dxs) {
     int r = blockIdx.y;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -2359,13 +2495,77 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
     double* V = v64 + (size_t)r * 3 * Np;
     double inv = 1.0 / dt;
     int slot = slot_of[(size_t)r * Np + i];
+    bool con = cflag && cflag[i];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double f = from_fixed(Fr[d * Np + slot]);
         double vel = vscale * V[d * Np + i] + fscale * im * f + noise * sim * gz[d];
         double delta = dt * vel;
-        X[d * Np + i] += delta;
-        V[d * Np + i] = inv * delta;
+        if (con) {
+            dxs[(size_t)r * 3 * Np + d * Np + i] = delta;
+        } else {
+            X[d * Np + i] += delta;
+            V[d * Np + i] = inv * delta;
+        }
+    }
+}
+
+// Position constraints of the Langevin step: SHAKE on one cluster of constrained atoms
+// per thread (Ryckaert, Ciccotti, Berendsen, J. Comput. Phys. 23, 327 (1977)): each
+// constraint in turn moves its two atoms along their bond vector before the step, in
+// inverse proportion to their masses, until every |r_ij|^2 is within tol of d_ij^2.
+// Then x = x', v = (x' - x)/dt.  Sets flag 8 if a cluster does not converge.
+constexpr int SHAKE_MAX_ATOMS = 16, SHAKE_MAX_CONS = 32, SHAKE_MAX_ITER = 500;
+__global__ __launch_bounds__(256) void k_constrain(
+    int Np, int ncl, const int* __restrict__ cl_start, const int* __restrict__ cl_atoms,
+    const int* __restrict__ con_start, const int* __restrict__ con_ij,
+    const double* __restrict__ con_d2, double tol, const double* __restrict__ invm,
+    const double* __restrict__ dxs, double* __restrict__ x64, double* __restrict__ v64,
+    const DevRecord* __restrict__ rec, int* __restrict__ flags) {
+    int r = blockIdx.y, q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncl) return;
+    double* X = x64 + (size_t)r * 3 * Np;
+    double* V = v64 + (size_t)r * 3 * Np;
+    const double* D = dxs + (size_t)r * 3 * Np;
+    int a0 = cl_start[q], na = cl_start[q + 1] - a0;
+    int c0 = con_start[q], nc = con_start[q + 1] - c0;
+    double xo[SHAKE_MAX_ATOMS][3], xn[SHAKE_MAX_ATOMS][3], im[SHAKE_MAX_ATOMS];
+    for (int k = 0; k < na; k++) {
+        int a = cl_atoms[a0 + k];
+        im[k] = invm[a];
+        for (int d = 0; d < 3; d++) {
+            xo[k][d] = X[d * Np + a];
+            xn[k][d] = xo[k][d] + D[d * Np + a];
+        }
+    }
+    bool done = false;
+    for (int it = 0; it < SHAKE_MAX_ITER && !done; it++) {
+        done = true;
+        for (int k = 0; k < nc; k++) {
+            int i = con_ij[2 * (c0 + k)], j = con_ij[2 * (c0 + k) + 1];
+            double d2 = con_d2[c0 + k];
+            double ro[3], rn[3];
+            for (int d = 0; d < 3; d++) {
+                ro[d] = xo[i][d] - xo[j][d];
+                rn[d] = xn[i][d] - xn[j][d];
+            }
+            double diff = d2 - dot3(rn, rn);
+            if (fabs(diff) > tol * d2) done = false;
+            double g = diff / (2.0 * (im[i] + im[j]) * dot3(ro, rn));
+            for (int d = 0; d < 3; d++) {
+                xn[i][d] += g * im[i] * ro[d];
+                xn[j][d] -= g * im[j] * ro[d];
+            }
+        }
+    }
+    if (!done) atomicOr(&flags[r], 8);
+    double inv = 1.0 / rec[r].dt;
+    for (int k = 0; k < na; k++) {
+        int a = cl_atoms[a0 + k];
+        for (int d = 0; d < 3; d++) {
+            V[d * Np + a] = inv * (xn[k][d] - xo[k][d]);
+            X[d * Np + a] = xn[k][d];
+        }
     }
 }
 

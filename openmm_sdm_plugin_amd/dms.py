@@ -76,10 +76,6 @@ class DMSTopology:
         return None if self.box is None else self.box.copy()
 
 
-def _empty(shape, dtype=np.float64):
-    return np.zeros(shape, dtype=dtype)
-
-
 @dataclasses.dataclass
 class DMSForceField:
     """The OpenMM System that the reference's createSystem builds, as arrays."""
@@ -426,3 +422,36 @@ class DesmondDMSFile:
             posre_h_hk=arr(ph_k, (-1, 3)), box=None if box is None else box.copy(),
             nonbonded_method=nonbondedMethod, cutoff=float(nonbondedCutoff),
             remove_cm_motion=bool(removeCMMotion))
+
+
+def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=None,
+                  ligand=None, eps_rf: float = 78.5, cutoff: Optional[float] = None,
+                  name: str = ""):
+    """An SDMSystem of the engine from a DMS force field: the nonbonded force with the
+    reaction field (dielectric eps_rf) in place of Ewald/PME, which the engine does not
+    have; bonds, angles and exclusions as SDMSystem arrays; torsions, impropers,
+    positional restraints, 1-4 exceptions and constraints as `terms`.  displacement:
+    (n, 3) nm per atom (zeros by default); ligand: the displaced atom indices."""
+    from .systems import SDMSystem  # local: systems does not depend on this module
+    if ff.box is None or ff.nonbonded_method in ("NoCutoff", "CutoffNonPeriodic"):
+        raise NotImplementedError("the engine evaluates periodic systems only "
+                                  "(CutoffPeriodic with the reaction field)")
+    if ff.opls:
+        raise NotImplementedError("geometric (OPLS) combining rules are not supported: the "
+                                  "pair kernels combine sigma arithmetically")
+    n = ff.n
+    x = np.asarray(positions, dtype=np.float64).reshape(n, 3)
+    v = np.zeros((n, 3)) if velocities is None else \
+        np.asarray(velocities, dtype=np.float64).reshape(n, 3)
+    disp = np.zeros((n, 3)) if displacement is None else \
+        np.asarray(displacement, dtype=np.float64).reshape(n, 3)
+    if ligand is None:
+        ligand = np.nonzero(np.any(disp != 0.0, axis=1))[0]
+    return SDMSystem(
+        box=ff.box.copy(), cutoff=float(ff.cutoff if cutoff is None else cutoff),
+        eps_rf=float(eps_rf), charge=ff.charge.copy(), sigma=ff.sigma.copy(),
+        epsilon=ff.epsilon.copy(), mass=ff.mass.copy(), exclusions=ff.exclusions.copy(),
+        bonds=ff.bonds.copy(), bond_r0=ff.bond_r0.copy(), bond_k=ff.bond_k.copy(),
+        angles=ff.angles.copy(), angle_t0=ff.angle_t0.copy(), angle_k=ff.angle_k.copy(),
+        displacement=disp, positions=x, velocities=v, periodic=True,
+        ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff)

@@ -106,6 +106,8 @@ class SDMSystem:
     name: str = ""
     # group-1 restraints (restraints.CentroidRestraint / AlignmentRestraint)
     restraints: list = dataclasses.field(default_factory=list)
+    # further force-field terms and constraints (a dms.DMSForceField), or None
+    terms: Optional[object] = None
 
     @property
     def n(self) -> int:
