@@ -16,7 +16,7 @@
 #define SDM_FIX_SCALE 4294967296.0       // 2^32
 #define SDM_FIX_LIMIT 2147483647.0       // |F| must stay below 2^31 kJ/mol/nm
 #define SDM_BOLTZ (1.380658e-23 * 6.0221367e23 / 1000.0)
-#define SDM_MAX_EXCL 16                  // exclusion partners per atom
+#define SDM_MAX_EXCL 32                  // exclusion partners per atom
 #define SDM_MAX_CLASSES 8                // distinct nonzero displacement vectors
 
 namespace sdm {

@@ -355,20 +355,21 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
     }
 #define SDM_PAIR_ARGS                                                                       \
     c->R, c->N, c->Np, c->NB, c->maxj, c->posq_s, c->par_s, c->orig_s, c->slot_of, c->tiles, \
-        c->tmask, c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->F, c->e_pair,   \
+        c->tmask, c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->max_ex, c->F, c->e_pair,   \
         c->flags, pc, c->g
 #define SDM_LAUNCH_PAIR(W)                                                                   \
     do {                                                                                     \
         dim3 gr((waves + W - 1) / W);                                                        \
         if (newton) {                                                                        \
-            if (count) k_pair<W, true, true><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);           \
-            else k_pair<W, false, true><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);                \
+            if (count) k_pair<W, true, true><<<gr, 64 * W, lds, s>>>(SDM_PAIR_ARGS);           \
+            else k_pair<W, false, true><<<gr, 64 * W, lds, s>>>(SDM_PAIR_ARGS);                \
         } else {                                                                             \
-            if (count) k_pair<W, true, false><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);          \
-            else k_pair<W, false, false><<<gr, 64 * W, 0, s>>>(SDM_PAIR_ARGS);               \
+            if (count) k_pair<W, true, false><<<gr, 64 * W, lds, s>>>(SDM_PAIR_ARGS);          \
+            else k_pair<W, false, false><<<gr, 64 * W, lds, s>>>(SDM_PAIR_ARGS);               \
         }                                                                                    \
     } while (0)
     bool newton = c->tune.pair_newton3 != 0;
+    size_t lds = (size_t)c->tune.pair_waves_per_group * std::max(1, c->max_ex) * 64 * sizeof(int);
     switch (c->tune.pair_waves_per_group) {
         case 2: SDM_LAUNCH_PAIR(2); break;
         case 4: SDM_LAUNCH_PAIR(4); break;
@@ -578,7 +579,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     int max_ex = 0;
     for (int i = 0; i < N; i++) {
         if (ecount[i + 1] > SDM_MAX_EXCL)
-            return fail(SDM_ERR_UNSUPPORTED, "more than 16 exclusions on one atom");
+            return fail(SDM_ERR_UNSUPPORTED, "more than 32 exclusions on one atom");
         max_ex = std::max(max_ex, ecount[i + 1]);
         ecount[i + 1] += ecount[i];
     }

@@ -643,10 +643,10 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
     const int* __restrict__ slot_of, const int* __restrict__ tiles,
     const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
     const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
-    const int* __restrict__ ex_start, const int* __restrict__ ex_list, long long* __restrict__ F,
-    double* __restrict__ e_pair, int* __restrict__ flags,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int max_ex,
+    long long* __restrict__ F, double* __restrict__ e_pair, int* __restrict__ flags,
     unsigned long long* __restrict__ pair_count, Geom g) {
-    __shared__ int exs_lds[WPW][SDM_MAX_EXCL][64];
+    extern __shared__ int exs_lds_dyn[];  // [WPW][max_ex][64] exclusion partners' slots
     // wave-uniform by construction; readfirstlane lets the compiler keep everything
     // derived from it (replica, block, tile, j index) in scalar registers
     int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -667,7 +667,8 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
         int ai = orig_s[(size_t)r * Np + islot];
         int b0 = ex_start[ai];
         nex = ex_start[ai + 1] - b0;
-        for (int k = 0; k < nex; k++) exs_lds[wv][k][lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+        for (int k = 0; k < nex; k++)
+            exs_lds_dyn[(wv * max_ex + k) * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
     }
     float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
     double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
@@ -684,7 +685,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
         unsigned long long am = tm[t];
         unsigned long long em = 0;
         for (int k = 0; k < nex; k++) {
-            int sl = exs_lds[wv][k][lane];
+            int sl = exs_lds_dyn[(wv * max_ex + k) * 64 + lane];
             if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
         }
         if (jb == ib) em |= NEWTON ? ((2ull << lane) - 1ull) : (1ull << lane);

@@ -38,52 +38,57 @@ def _min_image(d, box):
     return d - box * np.round(d / box)
 
 
+def _terms(ff, box):
+    """(group, atoms, energy of the term's atoms' positions) for every term."""
+    for (a, b, c, d), n, ph, k in zip(ff.torsions, ff.torsion_n, ff.torsion_phase, ff.torsion_k):
+        yield 1, (a, b, c, d), (lambda p, n=n, ph=ph, k=k:
+                                k * (1.0 + math.cos(n * dihedral(*p) - ph)))
+    for (a, b, c, d), p0, k in zip(ff.impropers, ff.improper_t0, ff.improper_k):
+        yield 1, (a, b, c, d), (lambda p, p0=p0, k=k: k * (dihedral(*p) - p0) ** 2)
+    for a, x0, hk, tol in zip(ff.posre_fb_atoms, ff.posre_fb_x0, ff.posre_fb_hk,
+                              ff.posre_fb_tol):
+        def fb(p, x0=x0, hk=hk, tol=tol):
+            dist = float(np.linalg.norm(_min_image(p[0] - x0, box)))
+            return hk * (dist - tol) ** 2 if dist >= tol else 0.0
+        yield 1, (a,), fb
+    for a, x0, hk in zip(ff.posre_h_atoms, ff.posre_h_x0, ff.posre_h_hk):
+        yield 1, (a,), (lambda p, x0=x0, hk=hk: float(np.sum(hk * (p[0] - x0) ** 2)))
+    for (i, j), qq, sg, eps in zip(ff.exception_pairs, ff.exception_qq, ff.exception_sigma,
+                                   ff.exception_epsilon):
+        def ex(p, qq=qq, sg=sg, eps=eps):
+            r = float(np.linalg.norm(p[0] - p[1]))
+            sr6 = (sg / r) ** 6
+            return 4 * eps * sr6 * (sr6 - 1.0) + KE * qq / r
+        yield 2, (i, j), ex
+
+
 def energy(x, ff, box):
     x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
-    e1 = e2 = 0.0
-    for (a, b, c, d), n, ph, k in zip(ff.torsions, ff.torsion_n, ff.torsion_phase, ff.torsion_k):
-        e1 += k * (1.0 + math.cos(n * dihedral(x[a], x[b], x[c], x[d]) - ph))
-    for (a, b, c, d), p0, k in zip(ff.impropers, ff.improper_t0, ff.improper_k):
-        e1 += k * (dihedral(x[a], x[b], x[c], x[d]) - p0) ** 2
-    for a, x0, hk, tol in zip(ff.posre_fb_atoms, ff.posre_fb_x0, ff.posre_fb_hk, ff.posre_fb_tol):
-        dist = float(np.linalg.norm(_min_image(x[a] - x0, box)))
-        if dist >= tol:
-            e1 += hk * (dist - tol) ** 2
-    for a, x0, hk in zip(ff.posre_h_atoms, ff.posre_h_x0, ff.posre_h_hk):
-        e1 += float(np.sum(hk * (x[a] - x0) ** 2))
-    for (i, j), qq, s, eps in zip(ff.exception_pairs, ff.exception_qq, ff.exception_sigma,
-                                  ff.exception_epsilon):
-        r = float(np.linalg.norm(x[i] - x[j]))
-        sr6 = (s / r) ** 6
-        e2 += 4 * eps * sr6 * (sr6 - 1.0) + KE * qq / r
-    return e1, e2
-
-
-def atoms_of(ff):
-    out = set()
-    for arr in (ff.torsions, ff.impropers, ff.exception_pairs):
-        out.update(int(a) for a in np.asarray(arr).reshape(-1))
-    out.update(int(a) for a in ff.posre_fb_atoms)
-    out.update(int(a) for a in ff.posre_h_atoms)
-    return sorted(out)
+    e = [0.0, 0.0, 0.0]
+    for grp, atoms, fn in _terms(ff, box):
+        e[grp] += fn([x[a] for a in atoms])
+    return e[1], e[2]
 
 
 def energy_forces(x, ff, box, h=1e-6):
-    """(E1, E2, F1, F2): group-1 and group-2 energies and forces (central differences)."""
-    x = np.array(x, dtype=np.float64).reshape(-1, 3)
-    e1, e2 = energy(x, ff, box)
-    f1, f2 = np.zeros_like(x), np.zeros_like(x)
-    for a in atoms_of(ff):
-        for d in range(3):
-            keep = x[a, d]
-            x[a, d] = keep + h
-            p1, p2 = energy(x, ff, box)
-            x[a, d] = keep - h
-            m1, m2 = energy(x, ff, box)
-            x[a, d] = keep
-            f1[a, d] = -(p1 - m1) / (2 * h)
-            f2[a, d] = -(p2 - m2) / (2 * h)
-    return e1, e2, f1, f2
+    """(E1, E2, F1, F2): group-1 and group-2 energies and forces, each term's force by
+    central differences over its own atoms."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    e = [0.0, 0.0, 0.0]
+    f = [None, np.zeros_like(x), np.zeros_like(x)]
+    for grp, atoms, fn in _terms(ff, box):
+        p = [x[a].copy() for a in atoms]
+        e[grp] += fn(p)
+        for m, a in enumerate(atoms):
+            for d in range(3):
+                keep = p[m][d]
+                p[m][d] = keep + h
+                ep = fn(p)
+                p[m][d] = keep - h
+                em = fn(p)
+                p[m][d] = keep
+                f[grp][a, d] -= (ep - em) / (2 * h)
+    return e[1], e[2], f[1], f[2]
 
 
 def _constrain_cluster(x_old, x_unc, im, pairs, d2, tol, max_iter):

@@ -109,7 +109,7 @@ def test_too_many_exclusions_is_unsupported():
     lib = _native.load()
     s = systems.make_small()
     s2 = s.with_coordinates(s.positions)
-    s2.exclusions = np.array([(0, j) for j in range(1, 18)], dtype=np.int32)
+    s2.exclusions = np.array([(0, j) for j in range(1, 34)], dtype=np.int32)
     desc, keep = _desc(s2)
     h = C.c_void_p()
     assert lib.sdm_create(C.byref(desc), 1, C.byref(h)) == _native.SDM_ERR_UNSUPPORTED

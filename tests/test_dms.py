@@ -216,3 +216,19 @@ def test_reference_examples(name):
         assert np.allclose(ff.box, [5.64589811, 6.32925716, 5.79653227])
         assert not ff.opls
     d.close()
+
+
+def test_fixture_matches_reader():
+    """tests/golden/temoa_g1_g4.npz (what the GPU tests run) is the reader's output."""
+    path = os.path.join(REF_EXAMPLES, "temoa-g1-g4.dms")
+    if not os.path.exists(path):
+        pytest.skip("the reference's example files are not present here")
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "temoa_g1_g4.npz"))
+    d = DesmondDMSFile(path)
+    ff = d.createSystem(nonbondedMethod="PME", nonbondedCutoff=1.0)
+    assert np.array_equal(z["positions"], d.getPositions())
+    assert np.array_equal(z["velocities"], d.getVelocities())
+    for k in ("mass", "charge", "sigma", "epsilon", "exclusions", "constraints", "constraint_r",
+              "torsions", "torsion_k", "exception_pairs", "exception_sigma", "posre_fb_x0"):
+        assert np.array_equal(z[k], getattr(ff, k)), k
+    d.close()
