@@ -97,6 +97,10 @@ typedef struct {
     double friction;         /* 1/ps */
     double step_size;        /* ps */
     uint64_t seed;           /* setRandomNumberSeed */
+    double m_alpha;          /* alpha slope while switching: alpha(lambda) = alpha +
+                                m_alpha*lambda, with dW/dalpha*m_alpha in the work
+                                increment.  0 (the reference, where alpha is constant and
+                                its work term is dropped, ReferenceSDMKernels.cpp:295) */
 } sdm_params;
 
 /* Observables of one replica for the last evaluated step

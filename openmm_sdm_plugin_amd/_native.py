@@ -55,7 +55,7 @@ class sdm_params(C.Structure):
         ("m_l1", C.c_double), ("m_l2", C.c_double), ("m_u0", C.c_double), ("m_w0", C.c_double),
         ("b_l1", C.c_double), ("b_l2", C.c_double), ("b_u0", C.c_double), ("b_w0", C.c_double),
         ("temperature", C.c_double), ("friction", C.c_double), ("step_size", C.c_double),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("m_alpha", C.c_double),
     ]
 
 

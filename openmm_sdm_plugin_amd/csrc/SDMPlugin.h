@@ -170,6 +170,9 @@ public:
     double getu0Slope() const { return p_.m_u0; }
     void setw0Slope(double v) { p_.m_w0 = v; }
     double getw0Slope() const { return p_.m_w0; }
+    // no counterpart in the reference (alpha is constant while switching there)
+    void setalphaSlope(double v) { p_.m_alpha = v; }
+    double getalphaSlope() const { return p_.m_alpha; }
     void setlambda1intercept(double v) { p_.b_l1 = v; }
     double getlambda1intercept() const { return p_.b_l1; }
     void setlambda2intercept(double v) { p_.b_l2 = v; }

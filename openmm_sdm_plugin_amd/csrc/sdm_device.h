@@ -31,6 +31,7 @@ struct DevState {
     double m_l1, m_l2, m_u0, m_w0, b_l1, b_l2, b_u0, b_w0;
     double temperature, friction, step_size;
     uint64_t seed;
+    double m_alpha;
     // clock (not part of sdm_params)
     double time;
     int64_t step;

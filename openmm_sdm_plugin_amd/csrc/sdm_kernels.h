@@ -2343,6 +2343,7 @@ __global__ __launch_bounds__(256) void k_scalar(
     double stepSize = S.step_size;
     double lambdac = S.lambda, dlambdac = 0.0, gamma = 0.0, wbcoeff = lambdac, w0coeff = 0.0;
     double lambda1 = lambdac, lambda2 = lambdac, alpha = 1.0, u0 = 0.0;
+    double alpha_now = S.alpha;  // alpha of this step (scheduled while switching)
     if (S.nonequilibrium == 1) {
         lambdac = S.time / S.noneq_tmax;
         S.lambda = lambdac;
@@ -2351,6 +2352,7 @@ __global__ __launch_bounds__(256) void k_scalar(
         S.u0 = S.m_u0 * lambdac + S.b_u0;
         S.w0coeff = S.m_w0 * lambdac + S.b_w0;
         dlambdac = stepSize / S.noneq_tmax;
+        alpha_now = S.alpha + S.m_alpha * lambdac;
     }
     int method = S.bias_method;
     if (method == 1) {
@@ -2360,7 +2362,7 @@ __global__ __launch_bounds__(256) void k_scalar(
     } else if (method == 2) {
         lambda1 = S.lambda1;
         lambda2 = S.lambda2;
-        alpha = S.alpha;
+        alpha = alpha_now;
         u0 = S.u0;
         w0coeff = S.w0coeff;
     }
@@ -2385,7 +2387,12 @@ __global__ __launch_bounds__(256) void k_scalar(
         double dwdl1 = -lee / alpha;
         double dwdl2 = be + lee / alpha;
         double dwdu0 = (lambda2 - lambda1) * (1.0 - inv_one_plus_exp(z));
-        double dwdlambda = dwdl1 * S.m_l1 + dwdl2 * S.m_l2 + dwdu0 * S.m_u0 + 1.0 * S.m_w0;
+        // dW/dalpha of the ILogistic bias: -(l2-l1) ln(1+e^z)/alpha^2 - (u'-u0) dW/du0/alpha
+        double dwdalpha = (alpha > 0) ? -(lambda2 - lambda1) * lee / (alpha * alpha) -
+                                            (be - u0) * dwdu0 / alpha
+                                      : 0.0;
+        double dwdlambda = dwdl1 * S.m_l1 + dwdl2 * S.m_l2 + dwdu0 * S.m_u0 + 1.0 * S.m_w0 +
+                           dwdalpha * S.m_alpha;
         S.work += dlambdac * dwdlambda;
     }
     double sp = bfp * fp;

@@ -121,6 +121,9 @@ class LangevinIntegratorSDM:
     def getu0Slope(self): return self._p.m_u0
     def setw0Slope(self, v): self._set("m_w0", float(v))
     def getw0Slope(self): return self._p.m_w0
+    # no counterpart in the reference (alpha is constant while switching there)
+    def setalphaSlope(self, v): self._set("m_alpha", float(v))
+    def getalphaSlope(self): return self._p.m_alpha
     def setlambda1intercept(self, v): self._set("b_l1", float(v))
     def getlambda1intercept(self): return self._p.b_l1
     def setlambda2intercept(self, v): self._set("b_l2", float(v))

@@ -78,6 +78,7 @@ class SDMParams:
     friction: float = 1.0
     step_size: float = 0.001
     seed: int = 12345
+    m_alpha: float = 0.0  # alpha(lambda) = alpha + m_alpha*lambda while switching
 
 
 @dataclasses.dataclass

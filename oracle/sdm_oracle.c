@@ -78,6 +78,8 @@ int orc_execute_scalars(orc_params* p, double E1, double E2, double Eb, double t
         p->w0coeff = p->m_w0 * lambdac + p->b_w0;
         dlambdac = stepSize / Ntmax;
     }
+    /* the reference keeps alpha constant; with a slope it follows lambda */
+    double alpha_now = p->alpha + (p->nonequilibrium == 1 ? p->m_alpha * lambdac : 0.0);
     int method = p->bias_method;
     if (method == ORC_QUADRATIC) {
         gamma = p->gamma;
@@ -86,7 +88,7 @@ int orc_execute_scalars(orc_params* p, double E1, double E2, double Eb, double t
     } else if (method == ORC_ILOGISTIC) {
         lambda1 = p->lambda1;
         lambda2 = p->lambda2;
-        alpha = p->alpha;
+        alpha = alpha_now;
         u0 = p->u0;
         w0coeff = p->w0coeff;
     }
@@ -112,8 +114,13 @@ int orc_execute_scalars(orc_params* p, double E1, double E2, double Eb, double t
         double dwdl2 = BindE + (log(ee) / alpha);
         double dwdu0 = (lambda2 - lambda1) * exp(-alpha * (BindE - u0)) / ee;
         double dwdw0 = 1;
+        /* d/dalpha of ((l2-l1)/alpha) ln(ee): the reference writes dwdalpha = 0 (:295),
+           which is exact only while alpha does not change with lambda */
+        double dwdalpha = (alpha > 0) ? -(lambda2 - lambda1) * log(ee) / (alpha * alpha) -
+                                            (BindE - u0) * dwdu0 / alpha
+                                      : 0.0;
         double dwdlambda = (dwdl1 * p->m_l1) + (dwdl2 * p->m_l2) + (dwdu0 * p->m_u0) +
-                           (dwdw0 * p->m_w0);
+                           (dwdw0 * p->m_w0) + dwdalpha * p->m_alpha;
         p->work += dlambdac * dwdlambda;
     }
     out->bind_e = BindE;

@@ -73,6 +73,7 @@ typedef struct {
     double noneq_tmax, work;
     double m_l1, m_l2, m_u0, m_w0, b_l1, b_l2, b_u0, b_w0;
     double temperature, friction, step_size;
+    double m_alpha; /* alpha(lambda) = alpha + m_alpha*lambda while switching (0: constant) */
 } orc_params;
 
 typedef struct {

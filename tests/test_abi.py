@@ -31,7 +31,7 @@ def test_library_exports_every_symbol():
 def test_struct_sizes_match_header_layout():
     # sdm_params: 2 ints, 11 doubles, int(+pad), 15 doubles... checked against the C
     # layout rule: every double is 8-aligned.
-    assert C.sizeof(_native.sdm_params) == 8 + 8 * 11 + 8 + 8 * 13 + 8
+    assert C.sizeof(_native.sdm_params) == 8 + 8 * 11 + 8 + 8 * 13 + 8 + 8
     assert C.sizeof(_native.sdm_energies) == 8 * 17 + 8 + 8
     assert C.sizeof(_native.sdm_tuning) == 28
     # 2 ints, 4 pointers, 5 doubles, 8 ints, 10 doubles

@@ -151,11 +151,14 @@ def test_step_parity_with_noise(hip_device):
     assert close(e["pot_energy"], rep.s.pot_energy)
 
 
-def test_nonequilibrium_schedule(hip_device):
+@pytest.mark.parametrize("m_alpha", [0.0, 0.04])
+def test_nonequilibrium_schedule(hip_device, m_alpha):
+    """m_alpha = 0 is the reference's switching; 0.04 also schedules alpha, whose work term
+    uses dW/dalpha (pinned by test_oracle_pins.test_work_alpha_slope_is_dW_dalpha)."""
     s = systems.make_small(frame=0)
     p = systems.params_config1()
     p.bias_method = 2
-    p.alpha = 0.1
+    p.alpha, p.m_alpha = 0.1, m_alpha
     p.nonequilibrium, p.noneq_tmax = 1, 0.004
     p.m_l1, p.m_l2, p.m_u0, p.m_w0 = 0.5, 0.8, 10.0, 1.0
     p.b_l1, p.b_l2, p.b_u0, p.b_w0 = 0.1, 0.1, 20.0, 0.0
