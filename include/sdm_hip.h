@@ -270,6 +270,12 @@ typedef struct {
    call in the reference, whose DMS reader adds these forces to the System. */
 int sdm_set_forcefield_terms(sdm_context* ctx, const sdm_forcefield_terms* terms);
 
+/* Centre-of-mass motion removal every `frequency` steps (0 = never, the default): the
+   mass-weighted mean velocity is subtracted from every atom with mass, before the
+   step's Langevin update, as OpenMM's CMMotionRemover does (which the reference's DMS
+   reader adds to every System, desmonddmsfile75.py:538-541). */
+int sdm_set_cm_motion_removal(sdm_context* ctx, int32_t frequency);
+
 /* Replaces the context's restraints (n = 0 removes them).  No counterpart call in the
    reference, where SDMUtils adds the forces to the System before the Context is built. */
 int sdm_set_restraints(sdm_context* ctx, const sdm_centroid_restraint* centroid,

@@ -120,7 +120,7 @@ EXPORTS = [
     "sdm_compute_components", "sdm_get_forces", "sdm_kinetic_energy", "sdm_set_u_history",
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
     "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
-    "sdm_set_forcefield_terms",
+    "sdm_set_forcefield_terms", "sdm_set_cm_motion_removal",
     "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
@@ -167,6 +167,7 @@ def load(path: str = LIB_PATH):
     lib.sdm_set_restraints.argtypes = [vp, C.POINTER(sdm_centroid_restraint), C.c_int32,
                                        C.POINTER(sdm_alignment_restraint), C.c_int32]
     lib.sdm_set_forcefield_terms.argtypes = [vp, C.POINTER(sdm_forcefield_terms)]
+    lib.sdm_set_cm_motion_removal.argtypes = [vp, C.c_int32]
     for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
               "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
               "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
@@ -174,7 +175,8 @@ def load(path: str = LIB_PATH):
               "sdm_get_energies", "sdm_clear_flags", "sdm_compute_components", "sdm_get_forces",
               "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
-              "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms"):
+              "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms",
+              "sdm_set_cm_motion_removal"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -261,6 +263,8 @@ class NativeContext:
             self.set_restraints(s.restraints)
         if getattr(s, "terms", None) is not None:
             self.set_forcefield_terms(s.terms)
+            if getattr(s.terms, "remove_cm_motion", False):
+                self.set_cm_motion_removal(1)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -327,6 +331,9 @@ class NativeContext:
                 a.offset[j] = float(r.offset[j])
             a.kfdispl, a.ktheta, a.kpsi = float(r.kfdispl), float(r.ktheta), float(r.kpsi)
         check(self._lib.sdm_set_restraints(self.h, ca, len(cen), aa, len(ali)))
+
+    def set_cm_motion_removal(self, frequency):
+        check(self._lib.sdm_set_cm_motion_removal(self.h, int(frequency)))
 
     def set_forcefield_terms(self, ff, constraint_tol=0.0):
         """Torsions, impropers, positional restraints, 1-4 exceptions and constraints of a

@@ -99,6 +99,7 @@ struct sdm_context {
     int* cflag = nullptr;
     double *con_d2 = nullptr, *dxs = nullptr;
     double con_tol = 1e-10;
+    int cm_every = 0;  // centre-of-mass motion removal interval (0: never)
     int *cand = nullptr, *ncand = nullptr;
     double4 *dA = nullptr, *dlig = nullptr, *dB = nullptr;
     DevState* state = nullptr;
@@ -448,6 +449,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
 void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullptr,
                   hipEvent_t pe1 = nullptr) {
     enqueue_eval(c, true, false, full, energy, pe0, pe1);
+    if (c->cm_every > 0)
+        k_cm_remove<<<c->R, 1024, 0, c->stream>>>(c->N, c->Np, c->cm_every, c->mass, c->v64,
+                                                   c->state);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
                                                            c->v64, c->rec, c->state, c->slot_of,
                                                            c->cflag, c->dxs);
@@ -863,6 +867,15 @@ static int set_constraints(sdm_context* c, const sdm_forcefield_terms* t) {
     CHK(upload(c, c->con_d2, con_d2.data(), con_d2.size()));
     CHK(upload(c, c->cflag, flag.data(), c->N));
     c->ncl = (int)members.size();
+    return SDM_OK;
+}
+
+int sdm_set_cm_motion_removal(sdm_context* c, int32_t frequency) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (frequency < 0) return fail(SDM_ERR_INVALID, "frequency must be >= 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->cm_every = frequency;
+    drop_graph(c);
     return SDM_OK;
 }
 

@@ -2518,6 +2518,41 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
     }
 }
 
+// centre-of-mass motion removal (one workgroup per replica) on steps that are multiples of
+// `every`: v -= sum(m v) / sum(m) for every atom with mass
+__global__ __launch_bounds__(1024) void k_cm_remove(int N, int Np, int every,
+                                                    const double* __restrict__ mass,
+                                                    double* __restrict__ v64,
+                                                    const DevState* __restrict__ state) {
+    __shared__ double red[4][1024];
+    int r = blockIdx.x, t = threadIdx.x;
+    if (state[r].step % every != 0) return;  // uniform over the workgroup
+    double* V = v64 + (size_t)r * 3 * Np;
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = t; i < N; i += 1024) {
+        double m = mass[i];
+        p[0] += m * V[i];
+        p[1] += m * V[Np + i];
+        p[2] += m * V[2 * Np + i];
+        p[3] += m;
+    }
+    for (int d = 0; d < 4; d++) red[d][t] = p[d];
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o)
+            for (int d = 0; d < 4; d++) red[d][t] += red[d][t + o];
+        __syncthreads();
+    }
+    double im = red[3][0] > 0.0 ? 1.0 / red[3][0] : 0.0;
+    double cx = red[0][0] * im, cy = red[1][0] * im, cz = red[2][0] * im;
+    for (int i = t; i < N; i += 1024) {
+        if (mass[i] == 0.0) continue;
+        V[i] -= cx;
+        V[Np + i] -= cy;
+        V[2 * Np + i] -= cz;
+    }
+}
+
 // Position constraints of the Langevin step: SHAKE on one cluster of constrained atoms
 // per thread (Ryckaert, Ciccotti, Berendsen, J. Comput. Phys. 23, 327 (1977)): each
 // constraint in turn moves its two atoms along their bond vector before the step, in

@@ -135,3 +135,26 @@ def test_constraints_hold_over_steps(hip_device):
     bad.exception_qq, bad.exception_sigma, bad.exception_epsilon = [0.1], [0.3], [0.2]
     with pytest.raises(_native.SDMError):
         ctx.set_forcefield_terms(bad)
+
+
+def test_cm_motion_removal(hip_device):
+    """With removal every step, a uniform drift is taken out before the update; the
+    internal forces add no net momentum, so the momentum after a noise-free,
+    friction-free step is zero (without removal it is M * drift)."""
+    s = systems.make_config1(frame=1)
+    p = systems.params_config1()
+    p.temperature, p.friction = 0.0, 0.0
+    m = np.asarray(s.mass)
+    v = np.asarray(s.velocities) + np.array([0.7, -0.4, 0.2])
+    out = []
+    for every in (0, 1):
+        ctx = _native.NativeContext(s, 1)
+        ctx.set_cm_motion_removal(every)
+        ctx.set_params(0, p)
+        ctx.set_positions(0, s.positions)
+        ctx.set_velocities(0, v)
+        ctx.step(1)
+        out.append((m[:, None] * ctx.get_velocities(0)).sum(0))
+    scale = float((m[:, None] * np.abs(v)).sum())
+    assert np.allclose(out[0], (m[:, None] * v).sum(0), atol=1e-6 * scale)
+    assert np.max(np.abs(out[1])) < 1e-9 * scale
