@@ -1371,6 +1371,10 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "sort_interval must be in [1, 100000]");
     if (t->energy_every_step != 0 && t->energy_every_step != 1)
         return fail(SDM_ERR_INVALID, "energy_every_step must be 0 or 1");
+    if (t->pair_newton3 != 0 && t->pair_newton3 != 1)
+        return fail(SDM_ERR_INVALID, "pair_newton3 must be 0 or 1");
+    if (t->use_graph != 0 && t->use_graph != 1)
+        return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;

@@ -357,3 +357,17 @@ def test_energy_on_last_step_of_call(hip_device):
             assert close(ea[k], eb[k], tol=1e-6), (k, ea[k], eb[k])
         assert ea["flags"] == eb["flags"] == 0
     assert a7["U0"] != a8["U0"]  # the single-step call evaluated its own energies
+
+
+@pytest.mark.parametrize("field,value", [("pair_newton3", 2), ("use_graph", 2),
+                                         ("pair_group", 4), ("pair_waves_per_group", 3),
+                                         ("sort_interval", 0), ("energy_every_step", -1)])
+def test_tuning_out_of_range_refused(hip_device, field, value):
+    """sdm_set_tuning refuses values outside each field's documented set and keeps the
+    previous tuning."""
+    s = systems.make_small()
+    ctx = _native.NativeContext(s, 1)
+    before = ctx.get_tuning()
+    with pytest.raises(_native.SDMError):
+        ctx.set_tuning(**{field: value})
+    assert ctx.get_tuning() == before
