@@ -1401,36 +1401,60 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // one trial: `steps` steps from the saved state after two unmeasured ones (the first
+    // sorts and, with use_graph, captures), then the saved state back
+    auto trial = [&](const sdm_tuning& t, double& ms_out) -> int {
+        CHK(sdm_set_tuning(c, &t));
+        CHK(sdm_step(c, 2));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        CHK(sdm_step(c, steps));
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        ms_out = ms;
+        CHK(upload(c, c->x64, x.data(), n3));
+        CHK(upload(c, c->v64, v.data(), n3));
+        c->since_sort = 0;
+        CHK(upload(c, c->state, st.data(), c->R));
+        CHK(upload(c, c->flags, fl.data(), c->R));
+        return SDM_OK;
+    };
     // pair kernels: block waves without / with Newton-3, grouped waves of 16 and of 8
     const int kernels[][2] = {{0, 0}, {0, 1}, {16, 1}, {8, 1}};
+    int rc = SDM_OK;
     for (double side : cell_sides)
         for (int w : waves)
-        for (const auto& kn : kernels) {
-            sdm_tuning t = c->tune;
-            t.pair_waves_per_group = w;
-            t.pair_group = kn[0];
-            t.pair_newton3 = kn[1];
-            t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
-            CHK(sdm_set_tuning(c, &t));
-            CHK(sdm_step(c, 2));
-            HIPCHK(hipEventRecord(e0, c->stream));
-            CHK(sdm_step(c, steps));
-            HIPCHK(hipEventRecord(e1, c->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
+            for (const auto& kn : kernels) {
+                if (rc != SDM_OK) break;
+                sdm_tuning t = c->tune;
+                t.pair_waves_per_group = w;
+                t.pair_group = kn[0];
+                t.pair_newton3 = kn[1];
+                t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
+                double ms = 0.0;
+                rc = trial(t, ms);
+                if (rc == SDM_OK && ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
             }
-            CHK(upload(c, c->x64, x.data(), n3));
-            CHK(upload(c, c->v64, v.data(), n3));
-            c->since_sort = 0;
-            CHK(upload(c, c->state, st.data(), c->R));
-            CHK(upload(c, c->flags, fl.data(), c->R));
+    // the winner with the step replayed from a captured graph and launched kernel by
+    // kernel: which is faster depends on the size of the step (graph replay of the
+    // three-stream step was measured slower than direct launches at 22 x 22,750 atoms)
+    if (rc == SDM_OK) {
+        sdm_tuning t = best;
+        t.use_graph = best.use_graph ? 0 : 1;
+        double ms = 0.0;
+        rc = trial(t, ms);
+        if (rc == SDM_OK && ms < best_ms) {
+            best_ms = ms;
+            best = t;
         }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc != SDM_OK) return rc;
     CHK(sdm_set_tuning(c, &best));
     if (chosen) *chosen = best;
     return SDM_OK;
