@@ -592,6 +592,30 @@ __device__ __forceinline__ void reduce_scatter24(const float (&v)[24], int lane,
     }
 }
 
+// Sum v[12] (4 j atoms x 3 components, index 3k+c) over the 64 lanes so that lanes
+// 16k..16k+15 end up holding the three components of j atom k: two halving stages
+// (v_permlane32_swap on lane bit 5, v_permlane16_swap on bit 4), then DPP sums over the
+// 16 lanes of each row (row_ror:8, the two quad swaps, row_half_mirror).
+__device__ __forceinline__ void reduce_scatter12(const float (&v)[12], float (&out)[3]) {
+    float w[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t]), __float_as_uint(v[t + 6]),
+                                                   false, false);
+        w[t] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[t]), __float_as_uint(w[t + 3]),
+                                                   false, false);
+        float y = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        y = dpp_add<0x128>(y, y);      // row_ror:8 = lane ^ 8 within 16
+        y = dpp_add<0xB1>(y, y);       // quad_perm [1,0,3,2] = lane ^ 1
+        y = dpp_add<0x4E>(y, y);       // quad_perm [2,3,0,1] = lane ^ 2
+        out[t] = dpp_add<0x141>(y, y); // row_half_mirror: lane i <- 7 - i within 8
+    }
+}
+
 // Newton-3 evaluation of one tile: j atoms in groups of 8; their forces are scattered
 // to the slot-ordered fixed-point accumulator F (3 planes of Np).  Slots past the last j
 // atom re-read the first one and are masked off.
@@ -865,13 +889,17 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
         // holds excluded pairs (both wave-uniform, compile-time inside the loop)
         auto chunks = [&](auto mic, auto exc) {
             constexpr bool MI = decltype(mic)::value, EX = decltype(exc)::value;
-            while (am) {
-                int jj[8];
+            // the next W j atoms of the tile: W = 8, or W = 4 for a remainder of at most
+            // 4 (about half the tiles end with one), whose chunk costs half as much
+            auto chunk = [&](auto wc) {
+                constexpr int W = decltype(wc)::value;
+                constexpr int SH = W == 8 ? 3 : 4;  // lanes (1 << SH) per j atom after the sum
+                int jj[W];
 #pragma unroll
-                for (int k = 0; k < 8; k++) jj[k] = next_bit(am);
-                float v[24];
+                for (int k = 0; k < W; k++) jj[k] = next_bit(am);
+                float v[3 * W];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
+                for (int k = 0; k < W; k++) {
                     int j = jj[k] < 0 ? jj[0] : jj[k];
                     float4 pj = buf[2 * j];
                     float2 qj = *(const float2*)&buf[2 * j + 1];
@@ -884,21 +912,22 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                     v[3 * k + 2] = -fr * dz;
                 }
                 float f[3];
-                reduce_scatter24(v, lane, f);
+                if constexpr (W == 8) reduce_scatter24(v, lane, f);
+                else reduce_scatter12(v, f);
                 if (ENERGY) {
-                    acc.e += (double)acc.ef;  // FP32 over 8 pairs, FP64 beyond
+                    acc.e += (double)acc.ef;  // FP32 over W pairs, FP64 beyond
                     acc.ef = 0.f;
                 }
-                int kq = lane >> 3;
+                int kq = lane >> SH;
                 int mine = jj[0];
 #pragma unroll
-                for (int q = 1; q < 8; q++) mine = (kq == q) ? jj[q] : mine;
+                for (int q = 1; q < W; q++) mine = (kq == q) ? jj[q] : mine;
                 // Distinct j atoms per lane and only this wave touches facc, so a plain
                 // read-add-write is exact.  It is written out because the compiler would
                 // otherwise put an s_waitcnt vmcnt(0) (a wait for the tile copy in
                 // flight) in front of any LDS access it cannot separate from that copy;
                 // the loads wait for themselves inside the statement.
-                if ((lane & 7) == 0 && mine >= 0) {
+                if ((lane & ((1 << SH) - 1)) == 0 && mine >= 0) {
                     unsigned addr = (unsigned)(size_t)&facc[wv][0][mine];
                     float o0, o1, o2;
                     asm volatile(
@@ -920,6 +949,10 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                         : "v"(addr), "v"(o0), "v"(o1), "v"(o2)
                         : "memory");
                 }
+            };
+            while (am) {
+                if (__popcll(am) <= 4) chunk(std::integral_constant<int, 4>());
+                else chunk(std::integral_constant<int, 8>());
             }
         };
         using T = std::true_type;
