@@ -616,6 +616,26 @@ __device__ __forceinline__ void reduce_scatter12(const float (&v)[12], float (&o
     }
 }
 
+// The same for v[6] (2 j atoms): lanes 32k..32k+31 end up holding j atom k.  Each sum
+// pairs lanes in the same order as reduce_scatter24 and reduce_scatter12 (lane ^ 32,
+// ^ 16, ^ 8, ^ 1, ^ 2, half mirror), so a j atom's sum does not depend on the width of
+// the chunk it was taken in.
+__device__ __forceinline__ void reduce_scatter6(const float (&v)[6], float (&out)[3]) {
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t]), __float_as_uint(v[t + 3]),
+                                                   false, false);
+        float w = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(w), __float_as_uint(w), false,
+                                                   false);
+        float y = __uint_as_float(q[0]) + __uint_as_float(q[1]);
+        y = dpp_add<0x128>(y, y);      // row_ror:8 = lane ^ 8 within 16
+        y = dpp_add<0xB1>(y, y);       // quad_perm [1,0,3,2] = lane ^ 1
+        y = dpp_add<0x4E>(y, y);       // quad_perm [2,3,0,1] = lane ^ 2
+        out[t] = dpp_add<0x141>(y, y); // row_half_mirror: lane i <- 7 - i within 8
+    }
+}
+
 // Newton-3 evaluation of one tile: j atoms in groups of 8; their forces are scattered
 // to the slot-ordered fixed-point accumulator F (3 planes of Np).  Slots past the last j
 // atom re-read the first one and are masked off.
@@ -889,11 +909,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
         // holds excluded pairs (both wave-uniform, compile-time inside the loop)
         auto chunks = [&](auto mic, auto exc) {
             constexpr bool MI = decltype(mic)::value, EX = decltype(exc)::value;
-            // the next W j atoms of the tile: W = 8, or W = 4 for a remainder of at most
-            // 4 (about half the tiles end with one), whose chunk costs half as much
+            // the next W j atoms of the tile, W = 8, 4 or 2: a chunk costs about 310, 166
+            // and 100 VALU instructions, so the tail of a tile is taken in narrower ones
             auto chunk = [&](auto wc) {
                 constexpr int W = decltype(wc)::value;
-                constexpr int SH = W == 8 ? 3 : 4;  // lanes (1 << SH) per j atom after the sum
+                constexpr int SH = W == 8 ? 3 : W == 4 ? 4 : 5;  // lanes per j atom: 1 << SH
                 int jj[W];
 #pragma unroll
                 for (int k = 0; k < W; k++) jj[k] = next_bit(am);
@@ -913,7 +933,8 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                 }
                 float f[3];
                 if constexpr (W == 8) reduce_scatter24(v, lane, f);
-                else reduce_scatter12(v, f);
+                else if constexpr (W == 4) reduce_scatter12(v, f);
+                else reduce_scatter6(v, f);
                 if (ENERGY) {
                     acc.e += (double)acc.ef;  // FP32 over W pairs, FP64 beyond
                     acc.ef = 0.f;
@@ -951,8 +972,10 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_n3(
                 }
             };
             while (am) {
-                if (__popcll(am) <= 4) chunk(std::integral_constant<int, 4>());
-                else chunk(std::integral_constant<int, 8>());
+                const int n = __popcll(am);
+                if (n >= 7) chunk(std::integral_constant<int, 8>());
+                else if (n >= 3) chunk(std::integral_constant<int, 4>());
+                else chunk(std::integral_constant<int, 2>());
             }
         };
         using T = std::true_type;
