@@ -812,8 +812,10 @@ __device__ __forceinline__ void wait_vmem3() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// At least 5 waves per SIMD (at most 102 VGPRs): the energy variant otherwise takes 97,
+// which leaves room for 4.
 template <int WPW, bool COUNT, bool ENERGY>
-__global__ __launch_bounds__(64 * WPW) void k_pair_n3(
+__global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_n3(
     int R, int N, int Np, int NB, int maxj, const float4* __restrict__ arec,
     const int* __restrict__ orig_s, const int* __restrict__ slot_of, const int* __restrict__ tiles,
     const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
