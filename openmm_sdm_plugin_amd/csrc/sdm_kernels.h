@@ -812,8 +812,8 @@ __device__ __forceinline__ void wait_vmem3() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// At least 5 waves per SIMD (at most 102 VGPRs): the energy variant otherwise takes 97,
-// which leaves room for 4.
+// At least 5 waves per SIMD (at most 96 VGPRs, allocated in granules of 8): the energy
+// variant otherwise takes 97, which leaves room for 4.
 template <int WPW, bool COUNT, bool ENERGY>
 __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_n3(
     int R, int N, int Np, int NB, int maxj, const float4* __restrict__ arec,
