@@ -147,6 +147,13 @@ typedef struct {
                                      the work and the forces are evaluated every step);
                                      1: on every step.  Block Newton-3 kernel only; the
                                      other pair kernels always evaluate them. */
+    int32_t pair_split;           /* block Newton-3 kernel only: 1, 2, 4 (the default) or
+                                     8 waves per i-block, each taking every pair_split-th
+                                     tile, so that short tile lists do not leave the GPU
+                                     waiting on long ones and small batches fill it.
+                                     Results differ only in rounding: each wave's i forces
+                                     and pair energies are summed on their own before they
+                                     are added up. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */

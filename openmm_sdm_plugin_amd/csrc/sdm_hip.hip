@@ -113,7 +113,7 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
-    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0};
+    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
     // captured steps: [with a full sort][with the pair energies]
@@ -334,8 +334,11 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
     if (c->tune.pair_group == 8) return launch_pair_grp_w<8>(c, count, pc);
     if (c->tune.pair_newton3) {
         size_t lds = (size_t)c->tune.pair_waves_per_group * c->max_ex * 64 * sizeof(int);
+        const int split = c->tune.pair_split;
+        waves *= split;
 #define SDM_N3_ARGS                                                                             \
-    c->R, c->N, c->Np, c->NB, c->maxj, c->arec, c->orig_s, c->slot_of, c->tiles, c->tmask,      \
+    c->R, c->N, c->Np, c->NB, c->maxj, split, c->arec, c->orig_s, c->slot_of, c->tiles,         \
+        c->tmask,                                                                           \
         c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->max_ex, c->F, c->e_pair, \
         c->flags, pc, c->g
 #define SDM_LAUNCH_N3(W)                                                                      \
@@ -378,6 +381,12 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
     }
 #undef SDM_PAIR_ARGS
 #undef SDM_LAUNCH_PAIR
+}
+
+// pair-energy slots per replica in e_pair: one per (i-block, wave of the block)
+int energy_slots(const sdm_context* c) {
+    bool split = c->tune.pair_group == 0 && c->tune.pair_newton3;
+    return c->NB * (split ? c->tune.pair_split : 1);
 }
 
 // evaluation of one step (everything before the Langevin update).  After the slot order
@@ -436,7 +445,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     (void)hipEventRecord(c->ev_join[1], s2);
     (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
     (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
-    k_scalar<<<R, 256, 0, s>>>(c->NB, c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
+    k_scalar<<<R, 256, 0, s>>>(energy_slots(c), c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
                                c->ncen + c->nali, c->e_t1, c->e_t2, c->ntb);
@@ -686,7 +695,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
-    A(e_pair, (size_t)R * NB); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
+    A(e_pair, (size_t)R * NB * 8); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
     A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
     A(dB, (size_t)R * c->maxc * 64); A(state, R); A(rec, R); A(flags, R);
@@ -1373,6 +1382,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "energy_every_step must be 0 or 1");
     if (t->pair_newton3 != 0 && t->pair_newton3 != 1)
         return fail(SDM_ERR_INVALID, "pair_newton3 must be 0 or 1");
+    if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8)
+        return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4 or 8");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1420,8 +1431,10 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
         CHK(upload(c, c->flags, fl.data(), c->R));
         return SDM_OK;
     };
-    // pair kernels: block waves without / with Newton-3, grouped waves of 16 and of 8
-    const int kernels[][2] = {{0, 0}, {0, 1}, {16, 1}, {8, 1}};
+    // pair kernels: block waves without Newton-3, with Newton-3 and 1, 2, 4 or 8 waves
+    // per i-block, grouped waves of 16 and of 8
+    const int kernels[][3] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 1, 4}, {0, 1, 8},
+                              {16, 1, 1}, {8, 1, 1}};
     int rc = SDM_OK;
     for (double side : cell_sides)
         for (int w : waves)
@@ -1431,6 +1444,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
                 t.pair_waves_per_group = w;
                 t.pair_group = kn[0];
                 t.pair_newton3 = kn[1];
+                t.pair_split = kn[2];
                 t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
                 double ms = 0.0;
                 rc = trial(t, ms);

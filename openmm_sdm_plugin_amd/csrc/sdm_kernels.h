@@ -794,7 +794,9 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 
 // ---------------------------------------------------------------------------
 // Block kernel, Newton-3, with LDS-staged j tiles.  Same work split as k_pair<.., true>
-// (one wave per i-block, half tile list, j wave-uniform), but the 64 slot records of
+// (one wave per i-block, half tile list, j wave-uniform; or `split` waves per i-block,
+// each taking every split-th tile, with their own i-force atomics and energy slot, for
+// batches too small to fill the GPU with one wave per block), but the 64 slot records of
 // each j-block are copied global -> LDS by DMA one tile ahead (no registers, no scalar
 // loads in the pair loop), and the j forces of a tile are summed in LDS and added to
 // the global accumulator once per tile with coalesced atomics.
@@ -816,7 +818,7 @@ __device__ __forceinline__ void wait_vmem3() {
 // variant otherwise takes 97, which leaves room for 4.
 template <int WPW, bool COUNT, bool ENERGY>
 __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_n3(
-    int R, int N, int Np, int NB, int maxj, const float4* __restrict__ arec,
+    int R, int N, int Np, int NB, int maxj, int split, const float4* __restrict__ arec,
     const int* __restrict__ orig_s, const int* __restrict__ slot_of, const int* __restrict__ tiles,
     const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
     const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
@@ -829,9 +831,12 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
     int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int w = xcd_linear_id(blockIdx.x, gridDim.x) * WPW + wv;
     int lane = threadIdx.x & 63;
-    if (w >= R * NB) return;
-    int r = w / NB;
-    int ib = w - r * NB;
+    if (w >= R * NB * split) return;
+    // split waves per (replica, i-block), wave `part` taking tiles part, part + split, ..
+    const int part = w % split;
+    const int rb = w / split;
+    int r = rb / NB;
+    int ib = rb - r * NB;
     const float4* A = arec + (size_t)r * Np * 2;
     unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
     int* exs = exs_dyn + wv * max_ex * 64;
@@ -881,7 +886,7 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         int jb = tl[t];
         unsigned long long am = tmk[t];
         const unsigned long long am0 = am;
-        if (t + 1 < nt) stage(tl[t + 1], std::integral_constant<int, 1 - B>());
+        if (t + split < nt) stage(tl[t + split], std::integral_constant<int, 1 - B>());
         unsigned long long em = 0;
         for (int k = 0; k < nex; k++) {
             int sl = exs[k * 64 + lane];
@@ -1029,11 +1034,11 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         else wait_vmem3();
     };
 
-    if (nt > 0) stage(tl[0], std::integral_constant<int, 0>());
+    if (part < nt) stage(tl[part], std::integral_constant<int, 0>());
     wait_vmem();  // the first tile, this lane's atom and the block boxes have arrived
-    for (int t = 0; t < nt; t += 2) {
+    for (int t = part; t < nt; t += 2 * split) {
         do_tile(t, std::integral_constant<int, 0>());
-        if (t + 1 < nt) do_tile(t + 1, std::integral_constant<int, 1>());
+        if (t + split < nt) do_tile(t + split, std::integral_constant<int, 1>());
     }
     if (COUNT) {
         double c = wave_sum(vi ? (double)acc.cnt : 0.0);
@@ -1049,7 +1054,7 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
     if (ENERGY) {
         double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
-        if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
+        if (lane == 0) e_pair[((size_t)r * NB + ib) * split + part] = es;
     }
 }
 
@@ -2355,8 +2360,9 @@ __device__ double softcore_dev(int method, double u, double umax, double a, doub
     return (umax - ub) * (zetap - 1.0) / (zetap + 1.0) + ub;
 }
 
+// ne: pair-energy slots per replica (i-blocks times the pair kernel's waves per block)
 __global__ __launch_bounds__(256) void k_scalar(
-    int NB, int nbw, int LB, const double* __restrict__ e_pair,
+    int ne, int nbw, int LB, const double* __restrict__ e_pair,
     const double* __restrict__ e_bond, const double* __restrict__ u_part,
     DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
     int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist,
@@ -2369,7 +2375,7 @@ __global__ __launch_bounds__(256) void k_scalar(
     // keep the values of the last evaluation that had them (nothing else depends on U0)
     double U0 = rec[r].U0;
     if (energy) {
-        for (int k = t; k < NB; k += 256) s0 += e_pair[(size_t)r * NB + k];
+        for (int k = t; k < ne; k += 256) s0 += e_pair[(size_t)r * ne + k];
         red[t] = s0;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {

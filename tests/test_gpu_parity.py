@@ -16,8 +16,10 @@ pytestmark = pytest.mark.gpu
 E_TOL = 1e-5
 F_TOL = 1e-4
 
-# pair kernels: block waves with / without Newton-3, grouped waves of 16 and 8 atoms
-KERNELS = {"block_n3": dict(pair_group=0, pair_newton3=1),
+# pair kernels: block waves with / without Newton-3 (with Newton-3 also 4 waves per
+# i-block), grouped waves of 16 and 8 atoms
+KERNELS = {"block_n3": dict(pair_group=0, pair_newton3=1, pair_split=1),
+           "block_n3_split4": dict(pair_group=0, pair_newton3=1, pair_split=4),
            "block_full": dict(pair_group=0, pair_newton3=0),
            "grp16": dict(pair_group=16, pair_newton3=1),
            "grp8": dict(pair_group=8, pair_newton3=1)}
@@ -272,7 +274,7 @@ def _translated_run(s, kernel, interval, nsteps, drift=30.0):
     return ctx, p
 
 
-@pytest.mark.parametrize("kernel", ["block_n3", "block_full", "grp16"])
+@pytest.mark.parametrize("kernel", ["block_n3", "block_n3_split4", "block_full", "grp16"])
 def test_components_between_sorts(hip_device, kernel):
     """Energies and forces evaluated in a kept slot order (sorted 10 steps and ~0.3 nm
     of translation earlier, atoms wrapped across the x faces since) match the oracle at
@@ -361,7 +363,8 @@ def test_energy_on_last_step_of_call(hip_device):
 
 @pytest.mark.parametrize("field,value", [("pair_newton3", 2), ("use_graph", 2),
                                          ("pair_group", 4), ("pair_waves_per_group", 3),
-                                         ("sort_interval", 0), ("energy_every_step", -1)])
+                                         ("sort_interval", 0), ("energy_every_step", -1),
+                                         ("pair_split", 3)])
 def test_tuning_out_of_range_refused(hip_device, field, value):
     """sdm_set_tuning refuses values outside each field's documented set and keeps the
     previous tuning."""
