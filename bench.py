@@ -114,7 +114,7 @@ def main():
         if a.profile_pair:
             # the same sequence as the benchmark up to its roofline timing
             if not a.no_autotune:
-                ctx.autotune(20)
+                ctx.autotune(40)
             ctx.step(a.warmup)
             ms = ctx.time_pair_kernel(a.profile_pair)
             print(json.dumps({"pair_kernel_ms_events": ms, "tuning": ctx.get_tuning()}))
@@ -136,7 +136,7 @@ def main():
                           "flags": [x["flags"] for x in e], "tuning": ctx.get_tuning(),
                           "calibration_bytes": src.numel() * 4, "calibration_sum": tot}))
         return
-    tuning = ctx.autotune(20) if not a.no_autotune else ctx.get_tuning()
+    tuning = ctx.autotune(40) if not a.no_autotune else ctx.get_tuning()
 
     # ---- batched ladder: warm-up ----
     ctx.step(a.warmup)
@@ -182,7 +182,7 @@ def main():
     one.set_positions(0, frames[0].positions)
     one.set_velocities(0, frames[0].velocities)
     if not a.no_autotune:
-        one.autotune(20)
+        one.autotune(200)
     one.step(a.warmup)
     one.energies()
     t1 = time.perf_counter()

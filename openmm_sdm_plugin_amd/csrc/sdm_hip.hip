@@ -283,10 +283,18 @@ void enqueue_order(sdm_context* c, bool full) {
 
 // tile lists of the block pair kernels (the grouped kernel finds its j-blocks itself)
 void enqueue_tiles(sdm_context* c) {
-    if (c->tune.pair_group == 0)
-        k_tiles<<<dim3(c->NB, c->R), 256, 0, c->stream>>>(
-            c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,
-            c->ntiles, c->list_r2, c->g, c->tune.pair_newton3, c->flags);
+    if (c->tune.pair_group != 0) return;
+    // One workgroup per (replica, i-block).  Its time is the serial path through it, so
+    // when the workgroups do not fill the GPU (fewer than 8 per CU) each gets 8 waves for
+    // the candidate masks, otherwise 4 (measured: one replica of 22,750 atoms +6%, 22
+    // replicas -1% with 8).
+#define SDM_TILES(NW)                                                                         \
+    k_tiles<NW><<<dim3(c->NB, c->R), 64 * NW, 0, c->stream>>>(                                \
+        c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,       \
+        c->ntiles, c->list_r2, c->g, c->tune.pair_newton3, c->flags)
+    if ((long)c->NB * c->R < 2048) SDM_TILES(8);
+    else SDM_TILES(4);
+#undef SDM_TILES
 }
 
 // slot order, block boxes and tile lists
@@ -1412,13 +1420,13 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // one trial: `steps` steps from the saved state after two unmeasured ones (the first
+    // one trial: n steps from the saved state after two unmeasured ones (the first
     // sorts and, with use_graph, captures), then the saved state back
-    auto trial = [&](const sdm_tuning& t, double& ms_out) -> int {
+    auto trial = [&](const sdm_tuning& t, int n, double& ms_out) -> int {
         CHK(sdm_set_tuning(c, &t));
         CHK(sdm_step(c, 2));
         HIPCHK(hipEventRecord(e0, c->stream));
-        CHK(sdm_step(c, steps));
+        CHK(sdm_step(c, n));
         HIPCHK(hipEventRecord(e1, c->stream));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1435,6 +1443,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     // per i-block, grouped waves of 16 and of 8
     const int kernels[][3] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 1, 4}, {0, 1, 8},
                               {16, 1, 1}, {8, 1, 1}};
+    std::vector<std::pair<double, sdm_tuning>> timed;
     int rc = SDM_OK;
     for (double side : cell_sides)
         for (int w : waves)
@@ -1447,20 +1456,29 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
                 t.pair_split = kn[2];
                 t.cell_cells_per_dim = std::max(1, (int)std::floor(L / side));
                 double ms = 0.0;
-                rc = trial(t, ms);
-                if (rc == SDM_OK && ms < best_ms) {
-                    best_ms = ms;
-                    best = t;
-                }
+                rc = trial(t, steps, ms);
+                if (rc == SDM_OK) timed.push_back({ms, t});
             }
-    // the winner with the step replayed from a captured graph and launched kernel by
-    // kernel: which is faster depends on the size of the step (graph replay of the
-    // three-stream step was measured slower than direct launches at 22 x 22,750 atoms)
-    if (rc == SDM_OK) {
+    // the three fastest again with trials three times as long (configurations within a
+    // few percent of each other are not told apart reliably by short trials), then the
+    // winner with the step replayed from a captured graph and launched kernel by kernel:
+    // which is faster depends on the size of the step (graph replay of the three-stream
+    // step was measured slower than direct launches at 22 x 22,750 atoms)
+    std::stable_sort(timed.begin(), timed.end(),
+                     [](const auto& p, const auto& q) { return p.first < q.first; });
+    for (size_t k = 0; k < timed.size() && k < 3 && rc == SDM_OK; k++) {
+        double ms = 0.0;
+        rc = trial(timed[k].second, 3 * steps, ms);
+        if (rc == SDM_OK && ms < best_ms) {
+            best_ms = ms;
+            best = timed[k].second;
+        }
+    }
+    if (rc == SDM_OK && best_ms < 1e300) {
         sdm_tuning t = best;
         t.use_graph = best.use_graph ? 0 : 1;
         double ms = 0.0;
-        rc = trial(t, ms);
+        rc = trial(t, 3 * steps, ms);
         if (rc == SDM_OK && ms < best_ms) {
             best_ms = ms;
             best = t;

@@ -244,11 +244,12 @@ __device__ __forceinline__ float max8(float v) {
 // (atoms outside all of them cannot have an in-cutoff partner in the i-block; the
 // union of the small boxes is a much tighter region than the block's box).  Half list:
 // in the diagonal tile an atom only counts for groups that start below it.
-// Four waves per i-block: the candidate j-blocks (box test) are listed in LDS, each
-// wave computes the atom masks of every fourth candidate, and wave 0 compacts the
-// non-empty ones in candidate order.
+// NW waves per i-block: the candidate j-blocks (box test) are listed in LDS, each
+// wave computes the atom masks of every NW-th candidate, and wave 0 compacts the
+// non-empty ones in candidate order.  Results do not depend on NW.
 constexpr int TILE_CAND = 256;  // candidates per round
-__global__ __launch_bounds__(256) void k_tiles(int N, int Np, int NB, int maxj,
+template <int NW>  // waves per i-block workgroup
+__global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int maxj,
                                                const float4* __restrict__ posq_s,
                                                const float4* __restrict__ bb_lo,
                                                const float4* __restrict__ bb_hi,
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(256) void k_tiles(int N, int Np, int NB, int maxj,
         __syncthreads();
         int nc = ncand_s;
         base = base_s;  // every wave continues from wave 0's position (uniform loop)
-        for (int c = wave; c < nc; c += 4) {
+        for (int c = wave; c < nc; c += NW) {
             unsigned long long am = mask_of(cand[c]);
             if (lane == 0) cmk[c] = am;
         }
