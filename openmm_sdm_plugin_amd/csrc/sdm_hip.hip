@@ -281,6 +281,10 @@ void enqueue_order(sdm_context* c, bool full) {
     }
 }
 
+// whether a launch of one workgroup per (replica, 64-atom block) leaves the GPU short of
+// work (fewer than 8 workgroups per CU)
+bool few_blocks(const sdm_context* c) { return (long)c->NB * c->R < 2048; }
+
 // tile lists of the block pair kernels (the grouped kernel finds its j-blocks itself)
 void enqueue_tiles(sdm_context* c) {
     if (c->tune.pair_group != 0) return;
@@ -292,7 +296,7 @@ void enqueue_tiles(sdm_context* c) {
     k_tiles<NW><<<dim3(c->NB, c->R), 64 * NW, 0, c->stream>>>(                                \
         c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,       \
         c->ntiles, c->list_r2, c->g, c->tune.pair_newton3, c->flags)
-    if ((long)c->NB * c->R < 2048) SDM_TILES(8);
+    if (few_blocks(c)) SDM_TILES(8);
     else SDM_TILES(4);
 #undef SDM_TILES
 }
@@ -401,8 +405,9 @@ int energy_slots(const sdm_context* c) {
 // and the tile lists the work forks: the pair kernel on the main stream, the bonded terms
 // and the displaced-state chain on two side streams; they join before the scalar stage.
 // (Forking before the tile lists, which the side work does not need, was measured
-// slower: the side kernels then slow the tile builder, which is on the critical path.)
-// pe0/pe1 (optional) are recorded around the pair kernel.
+// slower on 22 replicas, where the side kernels then slow the tile builder, which is on
+// the critical path, and no faster on one.)  pe0/pe1 (optional) are recorded around the
+// pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
                   hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
     hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
