@@ -799,8 +799,9 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 // each taking every split-th tile, with their own i-force atomics and energy slot, for
 // batches too small to fill the GPU with one wave per block), but the 64 slot records of
 // each j-block are copied global -> LDS by DMA one tile ahead (no registers, no scalar
-// loads in the pair loop), and the j forces of a tile are summed in LDS and added to
-// the global accumulator once per tile with coalesced atomics.
+// loads in the pair loop), and the j forces of a tile are kept in LDS (each j atom's
+// sum at its rank in the tile's mask: every j atom falls in exactly one chunk) and
+// added to the global accumulator once per tile with coalesced atomics.
 //   LDS per wave: two 2 KB tile buffers (separate objects, so reading one never waits
 //   for the copy into the other), 768 B of j-force sums, max_ex*256 B of exclusions.
 //   vmcnt discipline: the only vector memory operations inside the tile loop are the
@@ -853,9 +854,6 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         nex = ex_start[ai + 1] - b0;
         for (int k = 0; k < nex; k++) exs[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
     }
-    facc[wv][0][lane] = 0.f;
-    facc[wv][1][lane] = 0.f;
-    facc[wv][2][lane] = 0.f;
     float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
     double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
     PairAcc acc;
@@ -914,7 +912,9 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         const bool ex = __ballot(em != 0ull) != 0ull;
         acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
         // all j atoms of the tile, 8 at a time; MI: per-pair minimum image, EX: the tile
-        // holds excluded pairs (both wave-uniform, compile-time inside the loop)
+        // holds excluded pairs (both wave-uniform, compile-time inside the loop).  ent:
+        // entries (j atoms in mask order) taken so far, nvalid: entries of this chunk.
+        int ent = 0, nvalid = 0;
         auto chunks = [&](auto mic, auto exc) {
             constexpr bool MI = decltype(mic)::value, EX = decltype(exc)::value;
             // the next W j atoms of the tile, W = 8, 4 or 2: a chunk costs about 310, 166
@@ -947,43 +947,37 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
                     acc.e += (double)acc.ef;  // FP32 over W pairs, FP64 beyond
                     acc.ef = 0.f;
                 }
-                int kq = lane >> SH;
-                int mine = jj[0];
-#pragma unroll
-                for (int q = 1; q < W; q++) mine = (kq == q) ? jj[q] : mine;
-                // Distinct j atoms per lane and only this wave touches facc, so a plain
-                // read-add-write is exact.  It is written out because the compiler would
-                // otherwise put an s_waitcnt vmcnt(0) (a wait for the tile copy in
-                // flight) in front of any LDS access it cannot separate from that copy;
-                // the loads wait for themselves inside the statement.
-                if ((lane & ((1 << SH) - 1)) == 0 && mine >= 0) {
-                    unsigned addr = (unsigned)(size_t)&facc[wv][0][mine];
-                    float o0, o1, o2;
-                    asm volatile(
-                        "ds_read_b32 %0, %3\n\t"
-                        "ds_read_b32 %1, %3 offset:256\n\t"
-                        "ds_read_b32 %2, %3 offset:512\n\t"
-                        "s_waitcnt lgkmcnt(0)"
-                        : "=v"(o0), "=v"(o1), "=v"(o2)
-                        : "v"(addr)
-                        : "memory");
-                    o0 += f[0];
-                    o1 += f[1];
-                    o2 += f[2];
+                // Lanes (lane >> SH) << SH hold the sum of entry ent + (lane >> SH): every
+                // j atom of the tile is one entry of one chunk, taken in mask order, so
+                // its sum is written (not added) at its rank in the tile's mask.  Written
+                // out because the compiler would otherwise put an s_waitcnt vmcnt(0) (a
+                // wait for the tile copy in flight) in front of any LDS access it cannot
+                // separate from that copy.
+                const int kq = lane >> SH;
+                if ((lane & ((1 << SH) - 1)) == 0 && kq < nvalid) {
+                    unsigned addr = (unsigned)(size_t)&facc[wv][0][ent + kq];
                     asm volatile(
                         "ds_write_b32 %0, %1\n\t"
                         "ds_write_b32 %0, %2 offset:256\n\t"
                         "ds_write_b32 %0, %3 offset:512"
                         :
-                        : "v"(addr), "v"(o0), "v"(o1), "v"(o2)
+                        : "v"(addr), "v"(f[0]), "v"(f[1]), "v"(f[2])
                         : "memory");
                 }
             };
             while (am) {
                 const int n = __popcll(am);
-                if (n >= 7) chunk(std::integral_constant<int, 8>());
-                else if (n >= 3) chunk(std::integral_constant<int, 4>());
-                else chunk(std::integral_constant<int, 2>());
+                if (n >= 7) {
+                    nvalid = min(n, 8);
+                    chunk(std::integral_constant<int, 8>());
+                } else if (n >= 3) {
+                    nvalid = min(n, 4);
+                    chunk(std::integral_constant<int, 4>());
+                } else {
+                    nvalid = n;
+                    chunk(std::integral_constant<int, 2>());
+                }
+                ent += nvalid;
             }
         };
         using T = std::true_type;
@@ -1002,11 +996,13 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
             aqr2 += (double)acc.sqr2;
             aq += (double)acc.sq;
         }
-        // flush the tile's j forces: lane = j atom, coalesced fixed-point atomics (the
-        // sums are read and cleared the same way as they were added)
+        // flush the tile's j forces: lane = j atom, which reads its sum at its rank in
+        // the mask (bits of am0 below the lane), coalesced fixed-point atomics
         float fjx, fjy, fjz;
         {
-            unsigned addr = (unsigned)(size_t)&facc[wv][0][lane];
+            const int rank = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(am0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am0, 0u));
+            unsigned addr = (unsigned)(size_t)&facc[wv][0][rank];
             asm volatile(
                 "ds_read_b32 %0, %3\n\t"
                 "ds_read_b32 %1, %3 offset:256\n\t"
@@ -1014,14 +1010,6 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
                 "s_waitcnt lgkmcnt(0)"
                 : "=v"(fjx), "=v"(fjy), "=v"(fjz)
                 : "v"(addr)
-                : "memory");
-            float z = 0.f;
-            asm volatile(
-                "ds_write_b32 %0, %1\n\t"
-                "ds_write_b32 %0, %1 offset:256\n\t"
-                "ds_write_b32 %0, %1 offset:512"
-                :
-                : "v"(addr), "v"(z)
                 : "memory");
         }
         if (!COUNT && ((am0 >> lane) & 1ull)) {
