@@ -1465,30 +1465,25 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
                 if (rc == SDM_OK) timed.push_back({ms, t});
             }
     // the three fastest again with trials three times as long (configurations within a
-    // few percent of each other are not told apart reliably by short trials), then the
-    // winner with the step replayed from a captured graph and launched kernel by kernel:
-    // which is faster depends on the size of the step (graph replay of the three-stream
-    // step was measured slower than direct launches at 22 x 22,750 atoms)
+    // few percent of each other are not told apart reliably by short trials), each with
+    // the step replayed from a captured graph and launched kernel by kernel: which is
+    // faster depends on the size of the step, and it changes the ranking of the other
+    // settings (graph replay of the three-stream step was measured slower than direct
+    // launches at 22 x 22,750 atoms, where 8 waves per i-block won under replay and 4
+    // under direct launches)
     std::stable_sort(timed.begin(), timed.end(),
                      [](const auto& p, const auto& q) { return p.first < q.first; });
-    for (size_t k = 0; k < timed.size() && k < 3 && rc == SDM_OK; k++) {
-        double ms = 0.0;
-        rc = trial(timed[k].second, 3 * steps, ms);
-        if (rc == SDM_OK && ms < best_ms) {
-            best_ms = ms;
-            best = timed[k].second;
+    for (size_t k = 0; k < timed.size() && k < 3 && rc == SDM_OK; k++)
+        for (int graph = 0; graph < 2 && rc == SDM_OK; graph++) {
+            sdm_tuning t = timed[k].second;
+            t.use_graph = graph;
+            double ms = 0.0;
+            rc = trial(t, 3 * steps, ms);
+            if (rc == SDM_OK && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
         }
-    }
-    if (rc == SDM_OK && best_ms < 1e300) {
-        sdm_tuning t = best;
-        t.use_graph = best.use_graph ? 0 : 1;
-        double ms = 0.0;
-        rc = trial(t, 3 * steps, ms);
-        if (rc == SDM_OK && ms < best_ms) {
-            best_ms = ms;
-            best = t;
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != SDM_OK) return rc;
