@@ -13,9 +13,10 @@ elapsed).  As in the reference, where the potential energy of a step(n) call is 
 after it, the pair energies are evaluated on the last step of each call; u, W, sp, the
 work and the forces on every step.  Also reported: the same ladder with the pair energies
 on every step, the single-replica plain path (R=1) steps/s and ns/day, the batched/plain
-ratio, the roofline of the pair kernel as the steps run it (forces only; FP32 VALU, 42
-FLOP per in-cutoff pair) timed live with HIP events on the library stream, and the FP64
-CPU baseline.
+ratio, the 8-replica ladder of the 99,920-atom relative-binding system (two guests
+displaced by +-3 nm, rc 1.2 nm), the roofline of the pair kernel as the steps run it
+(forces only; FP32 VALU, 42 FLOP per in-cutoff pair) timed live with HIP events on the
+library stream, and the FP64 CPU baseline.
 """
 import argparse
 import json
@@ -41,6 +42,9 @@ def parse():
     ap.add_argument("--single-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 8-replica 99,920-atom ladder")
+    ap.add_argument("--large-steps", type=int, default=300)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
@@ -189,6 +193,35 @@ def main():
     one.step(a.single_steps)
     one.energies()
     single = a.single_steps / (time.perf_counter() - t1)
+    del one
+
+    # ---- the large relative-binding ladder: 8 replicas of the 99,920-atom system (two
+    # guests displaced by +-3 nm, rc 1.2 nm), all starting from one frame ----
+    large = None
+    if not a.no_large:
+        s5 = systems.make_config5(frame=0)
+        p5 = systems.params_config5(8)
+        big = _native.NativeContext(s5, 8)
+        for r in range(8):
+            big.set_params(r, p5[r])
+            big.set_positions(r, s5.positions)
+            big.set_velocities(r, s5.velocities)
+        tuning5 = big.autotune(10) if not a.no_autotune else big.get_tuning()
+        big.step(a.warmup)
+        big.energies()
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        big.step(a.large_steps)
+        e5 = big.energies()
+        torch.cuda.synchronize()
+        dt5 = time.perf_counter() - t5
+        ok = ok and not any(x["flags"] for x in e5)
+        large = {"workload": "8-replica lambda ladder (0.1-0.45) of the 99,920-atom system, "
+                             "10 nm box, rc 1.2 nm, replicas from one frame",
+                 "aggregate_steps_per_s": 8 * a.large_steps / dt5,
+                 "ms_per_step": 1e3 * dt5 / a.large_steps, "steps": a.large_steps,
+                 "tuning": tuning5}
+        del big
 
     cpu = None
     if not a.no_cpu:
@@ -217,6 +250,7 @@ def main():
         },
         "single_replica_steps_per_s": single,
         "single_replica_ns_per_day": single * 0.001 * 86400.0 / 1000.0,
+        "large_ladder": large,
         "steps_per_s_energy_every_step": agg_all,
         "batched_over_plain": agg / single,
         "all_finite_no_flags": ok,

@@ -61,6 +61,9 @@ struct sdm_context {
     float *qs = nullptr, *sh = nullptr, *se = nullptr;
     double *q64 = nullptr, *sig64 = nullptr, *eps64 = nullptr, *invm = nullptr, *mass = nullptr;
     int *ex_start = nullptr, *ex_list = nullptr, *cls = nullptr, *lig = nullptr, *cellrank = nullptr;
+    // blocks of the displaced-atom list: entries lb_start[b] .. + lb_count[b] (<= 64),
+    // all of one displacement class, so a block's bounding box covers one group of atoms
+    int *lb_start = nullptr, *lb_count = nullptr;
     double* cdispl = nullptr;
     int* bond_atoms = nullptr;
     double *bond_r0 = nullptr, *bond_k = nullptr;
@@ -401,22 +404,31 @@ int energy_slots(const sdm_context* c) {
     return c->NB * (split ? c->tune.pair_split : 1);
 }
 
-// evaluation of one step (everything before the Langevin update).  After the slot order
-// and the tile lists the work forks: the pair kernel on the main stream, the bonded terms
-// and the displaced-state chain on two side streams; they join before the scalar stage.
+// evaluation of one step (everything before the Langevin update).  With `fork` (only
+// while a step graph is captured) the work forks after the slot order and the tile
+// lists: the pair kernel on the main stream, the bonded terms and the displaced-state
+// chain on two side streams; they join before the scalar stage.  Without it everything
+// runs in order on the main stream.  Direct launches do not fork: forked by events
+// between streams, they gave wrong trajectories for a few percent of contexts (side
+// kernels read main-stream data from before the fork), depending on how the process's
+// streams were spread over the hardware queues, while captured graphs, whose
+// dependencies are graph edges, and single-stream launches were exact in every check.
 // (Forking before the tile lists, which the side work does not need, was measured
 // slower on 22 replicas, where the side kernels then slow the tile builder, which is on
 // the critical path, and no faster on one.)  pe0/pe1 (optional) are recorded around the
 // pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
-                  hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
-    hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
+                  bool fork, hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
+    hipStream_t s = c->stream;
+    hipStream_t s1 = fork ? c->side[0] : s, s2 = fork ? c->side[1] : s;
     int N = c->N, Np = c->Np, R = c->R;
     enqueue_order(c, full);
     enqueue_tiles(c);
-    (void)hipEventRecord(c->ev_fork, s);
-    (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
-    (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
+    if (fork) {
+        (void)hipEventRecord(c->ev_fork, s);
+        (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
+        (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
+    }
     if (c->nbond + c->nang > 0)
         k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
                                                   c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
@@ -438,11 +450,13 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
                                     c->g, c->flags);
         k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, s2>>>(
-            N, Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
+            c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
             nullptr);
         k_deltaA_red<<<dim3(c->LB, R), 64, 0, s2>>>(
-            Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
+            Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
+            c->sig64, c->eps64, c->ex_start,
             c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g, nullptr);
         k_deltaB<<<dim3(c->maxc, R), 64, 0, s2>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
                                                   c->sig64, c->eps64, c->ex_start, c->ex_list,
@@ -454,10 +468,12 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     energy = energy || c->tune.pair_group != 0 || !c->tune.pair_newton3;
     enqueue_pair(c, false, energy);
     if (pe1) (void)hipEventRecord(pe1, s);
-    (void)hipEventRecord(c->ev_join[0], s1);
-    (void)hipEventRecord(c->ev_join[1], s2);
-    (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
-    (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
+    if (fork) {
+        (void)hipEventRecord(c->ev_join[0], s1);
+        (void)hipEventRecord(c->ev_join[1], s2);
+        (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
+        (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
+    }
     k_scalar<<<R, 256, 0, s>>>(energy_slots(c), c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
@@ -468,9 +484,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
                                                        c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
-void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullptr,
+void enqueue_step(sdm_context* c, bool full, bool energy, bool fork, hipEvent_t pe0 = nullptr,
                   hipEvent_t pe1 = nullptr) {
-    enqueue_eval(c, true, false, full, energy, pe0, pe1);
+    enqueue_eval(c, true, false, full, energy, fork, pe0, pe1);
     if (c->cm_every > 0)
         k_cm_remove<<<c->R, 1024, 0, c->stream>>>(c->N, c->Np, c->cm_every, c->mass, c->v64,
                                                    c->state);
@@ -493,7 +509,7 @@ int build_graph(sdm_context* c, bool full, bool energy) {
     hipGraph_t& graph = c->graph[full][energy];
     hipGraphExec_t& gexec = c->gexec[full][energy];
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, full, energy);
+    enqueue_step(c, full, energy, true);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -644,6 +660,25 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         cls[i] = k;
         lig.push_back(i);
     }
+    // grouped by class, in atom order within a class; blocks of at most 64 entries that
+    // never span two classes (with two guests, a block spanning both would have a
+    // bounding box over both, and the displaced-state kernels cull candidate atoms
+    // against that box)
+    std::stable_sort(lig.begin(), lig.end(), [&](int a, int b) { return cls[a] < cls[b]; });
+    std::vector<int> lbs, lbn;
+    for (size_t k0 = 0; k0 < lig.size();) {
+        size_t k1 = k0;
+        while (k1 < lig.size() && cls[lig[k1]] == cls[lig[k0]]) k1++;
+        for (size_t o = k0; o < k1; o += 64) {
+            lbs.push_back((int)o);
+            lbn.push_back((int)std::min<size_t>(64, k1 - o));
+        }
+        k0 = k1;
+    }
+    if (lbs.empty()) {
+        lbs.push_back(0);
+        lbn.push_back(0);
+    }
 
     sdm_context* c = new sdm_context();
     int rc = SDM_OK;
@@ -665,7 +700,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     c->NB = c->Np / 64;
     c->R = n_replicas;
     c->L = (int)lig.size();
-    c->LB = std::max(1, (c->L + 63) / 64);
+    c->LB = (int)lbs.size();
     c->ncls = (int)cd.size() / 3 - 1;
     c->nbond = sys->n_bonds;
     c->nang = sys->n_angles;
@@ -700,6 +735,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(qs, Np); A(sh, Np); A(se, Np); A(q64, Np); A(sig64, Np); A(eps64, Np);
     A(invm, Np); A(mass, Np); A(ex_start, N_ + 1); A(ex_list, elist.size());
     A(cls, Np); A(lig, std::max(1, c->L)); A(cdispl, cd.size());
+    A(lb_start, lbs.size()); A(lb_count, lbn.size());
     A(bond_atoms, 2 * std::max(1, c->nbond)); A(bond_r0, std::max(1, c->nbond));
     A(bond_k, std::max(1, c->nbond)); A(ang_atoms, 3 * std::max(1, c->nang));
     A(ang_t0, std::max(1, c->nang)); A(ang_k, std::max(1, c->nang));
@@ -734,6 +770,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
             (rc = upload(c, c->orig_s, orig_pad.data(), RNp)))
             return bail(rc);
         if (c->L > 0 && (rc = upload(c, c->lig, lig.data(), lig.size()))) return bail(rc);
+        if ((rc = upload(c, c->lb_start, lbs.data(), lbs.size()))) return bail(rc);
+        if ((rc = upload(c, c->lb_count, lbn.data(), lbn.size()))) return bail(rc);
         if (c->nbond > 0 &&
             ((rc = upload(c, c->bond_atoms, sys->bond_atoms, 2 * (size_t)c->nbond)) ||
              (rc = upload(c, c->bond_r0, sys->bond_r0, c->nbond)) ||
@@ -1170,7 +1208,7 @@ int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
 
 int sdm_compute(sdm_context* c) {
     if (!c) return fail(SDM_ERR_INVALID, "null context");
-    enqueue_eval(c, false, false, take_order(c, false), true);
+    enqueue_eval(c, false, false, take_order(c, false), true, false);
     c->components_valid = false;
     return check_launch();
 }
@@ -1189,7 +1227,7 @@ int sdm_step(sdm_context* c, int32_t nsteps) {
             if (!c->gexec[full][energy]) CHK(build_graph(c, full, energy));
             HIPCHK(hipGraphLaunch(c->gexec[full][energy], c->stream));
         } else {
-            enqueue_step(c, full, energy);
+            enqueue_step(c, full, energy, false);
         }
     }
     return check_launch();
@@ -1228,7 +1266,7 @@ int sdm_compute_components(sdm_context* c) {
     }
     k_fill_ll<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->Fb_sep, n3);
     k_fill_d<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->dFd, n3);
-    enqueue_eval(c, false, true, take_order(c, false), true);
+    enqueue_eval(c, false, true, take_order(c, false), true, false);
     if (c->L > 0)
         k_scatter_dF<<<dim3(c->maxc + c->LB, c->R), 64, 0, c->stream>>>(
             c->N, c->Np, c->L, c->maxc, c->lig, c->cls, c->orig_s, c->cand, c->ncand, c->dB,
@@ -1321,11 +1359,13 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
                                       c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
                                       c->list_r2, c->g, c->flags);
         k_deltaA<<<dim3(c->maxc, c->LB, c->R), 64, 0, s>>>(
-            c->N, c->Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
+            c->q64, c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
             c->counters + c->R);
         k_deltaA_red<<<dim3(c->LB, c->R), 64, 0, s>>>(
-            c->Np, c->L, c->LB, c->lig, c->cls, c->cdispl, c->q64, c->sig64, c->eps64,
+            c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
+            c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g,
             c->counters + c->R);
     }

@@ -805,16 +805,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 //   LDS per wave: two 2 KB tile buffers (separate objects, so reading one never waits
 //   for the copy into the other), 768 B of j-force sums, max_ex*256 B of exclusions.
 //   vmcnt discipline: the only vector memory operations inside the tile loop are the
-//   next tile's copy and the three flush atomics that end each tile, so waiting for
-//   vmcnt <= 3 after the atomics means the copy has landed.
+//   next tile's copy and the three flush atomics that end each tile.  Loads and
+//   stores/atomics share vmcnt but need not retire in issue order, so the copy is
+//   waited for with vmcnt(0) before the atomics, not with vmcnt(3) after them.
 //   ENERGY = false: forces only; e_pair is not written.
 // ---------------------------------------------------------------------------
-// s_waitcnt vmcnt(3) alone
-__device__ __forceinline__ void wait_vmem3() {
-    __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(0x0F73);
-    __builtin_amdgcn_sched_barrier(0);
-}
 
 // At least 5 waves per SIMD (at most 96 VGPRs, allocated in granules of 8): the energy
 // variant otherwise takes 97, which leaves room for 4.
@@ -1012,15 +1007,16 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
                 : "v"(addr)
                 : "memory");
         }
+        // the next tile's records have landed (and the previous tile's atomics, issued a
+        // tile earlier, are done): waited for before this tile's atomics, because a load
+        // and later atomics share vmcnt but may retire out of order
+        wait_vmem();
         if (!COUNT && ((am0 >> lane) & 1ull)) {
             int s = jb * 64 + lane;
             atomicAdd(&Fr[s], fixed_from_float(fjx, &bad));
             atomicAdd(&Fr[Np + s], fixed_from_float(fjy, &bad));
             atomicAdd(&Fr[2 * Np + s], fixed_from_float(fjz, &bad));
         }
-        // the next tile's records have landed (counting mode issues no atomics)
-        if (COUNT) wait_vmem();
-        else wait_vmem3();
     };
 
     if (part < nt) stage(tl[part], std::integral_constant<int, 0>());
@@ -2171,9 +2167,12 @@ __device__ __forceinline__ bool excl_with(const int* __restrict__ ex_start,
     return any;
 }
 
-// lanes = displaced atoms (ligand block lb), j = undisplaced atoms of candidate block c.
+// lanes = displaced atoms (ligand block lb: entries lb_start[lb] .. + lb_count[lb] of
+// lig, all of one displacement class), j = undisplaced atoms of candidate block c.
 __global__ __launch_bounds__(64) void k_deltaA(
-    int N, int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ cls,
+    int N, int Np, int L, int LB, const int* __restrict__ lig,
+    const int* __restrict__ lb_start, const int* __restrict__ lb_count,
+    const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ q64,
     const double* __restrict__ sig64, const double* __restrict__ eps64,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
@@ -2184,16 +2183,16 @@ __global__ __launch_bounds__(64) void k_deltaA(
     if (c >= ncand[r]) return;  // uniform over the workgroup
     __shared__ AtomTile64 T;
     const double* X = x64 + (size_t)r * 3 * Np;
-    int li = lb * 64 + lane;
-    bool v = li < L;
-    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
+    int li = lb_start[lb] + lane;
+    bool v = lane < lb_count[lb];
+    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     int jb = cand[(size_t)r * maxc + c];
     int jbeg = jb * 64, jn = min(64, N - jbeg);
     if (lane < jn)
         stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + jbeg + lane], cls, q64, sig64, eps64);
     bool exI = v && excl_with(ex_start, ex_list, cls, I.idx, false);
-    // boxes of this block of displaced atoms at x and at x+d (lanes past L repeat atom 0
-    // of the ligand, which only widens them): j atoms beyond the cutoff of both boxes
+    // boxes of this block of displaced atoms at x and at x+d (lanes past the block's
+    // count repeat its first atom): j atoms beyond the cutoff of both boxes
     // have no pair inside it at either geometry and are skipped (their terms are zero)
     double b0lx = wave_min_d(I.x), b0ly = wave_min_d(I.y), b0lz = wave_min_d(I.z);
     double b0hx = wave_max_d(I.x), b0hy = wave_max_d(I.y), b0hz = wave_max_d(I.z);
@@ -2228,7 +2227,8 @@ __global__ __launch_bounds__(64) void k_deltaA(
 // per displaced atom: candidate partials in order, plus pairs with displaced atoms of a
 // different class (D x D); writes dlig and the ligand block's u.
 __global__ __launch_bounds__(64) void k_deltaA_red(
-    int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ cls,
+    int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ lb_start,
+    const int* __restrict__ lb_count, const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ q64,
     const double* __restrict__ sig64, const double* __restrict__ eps64,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
@@ -2237,9 +2237,9 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     Geom g, unsigned long long* __restrict__ pair_count) {
     int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
     const double* X = x64 + (size_t)r * 3 * Np;
-    int li = lb * 64 + lane;
-    bool v = li < L;
-    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[0], cls, cdispl, q64, sig64, eps64);
+    int li = lb_start[lb] + lane;
+    bool v = lane < lb_count[lb];
+    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int nc = ncand[r];
 #pragma unroll 8
