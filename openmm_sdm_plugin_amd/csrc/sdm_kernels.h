@@ -998,12 +998,14 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
             const int rank = __builtin_amdgcn_mbcnt_hi(
                 (unsigned)(am0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am0, 0u));
             unsigned addr = (unsigned)(size_t)&facc[wv][0][rank];
+            // early-clobber outputs: the first load's result lands while the other two
+            // still read the address, so no output may share the address register
             asm volatile(
                 "ds_read_b32 %0, %3\n\t"
                 "ds_read_b32 %1, %3 offset:256\n\t"
                 "ds_read_b32 %2, %3 offset:512\n\t"
                 "s_waitcnt lgkmcnt(0)"
-                : "=v"(fjx), "=v"(fjy), "=v"(fjz)
+                : "=&v"(fjx), "=&v"(fjy), "=&v"(fjz)
                 : "v"(addr)
                 : "memory");
         }

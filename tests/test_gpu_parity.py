@@ -315,6 +315,27 @@ def test_sort_interval_trajectory(hip_device):
     assert ea["flags"] == eb["flags"] == 0
 
 
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_steps_agree_across_contexts(hip_device, use_graph):
+    """The same 9 steps in many contexts of one process, created one after another with
+    0-4 others alive (so the process's streams take many spreads over the hardware
+    queues), give bit-identical trajectories: every launch keeps the step's data
+    dependencies whatever the stream mapping."""
+    s = systems.make_config1(frame=0)
+    ref = None
+    for k, nidle in enumerate((0, 2, 4, 1, 3, 0, 4, 2, 1, 3, 0, 2)):
+        idle = [_translated_run(s, "block_n3", interval=4, nsteps=0)[0] for _ in range(nidle)]
+        ctx, _ = _translated_run(s, "block_n3", interval=4, nsteps=0)
+        ctx.set_tuning(use_graph=use_graph)
+        ctx.step(9)
+        x = ctx.get_positions(0)
+        assert ctx.energies()[0]["flags"] == 0
+        if ref is None:
+            ref = x
+        assert np.array_equal(x, ref), (k, nidle, float(np.max(np.abs(x - ref))))
+        del ctx, idle
+
+
 def test_energy_calls_keep_step_order(hip_device):
     """Energy evaluations between steps do not change the slot orders the steps use."""
     s = systems.make_config1(frame=0)
@@ -359,6 +380,33 @@ def test_energy_on_last_step_of_call(hip_device):
             assert close(ea[k], eb[k], tol=1e-6), (k, ea[k], eb[k])
         assert ea["flags"] == eb["flags"] == 0
     assert a7["U0"] != a8["U0"]  # the single-step call evaluated its own energies
+
+
+@pytest.mark.parametrize("use_graph,split", [(0, 1), (1, 8)])
+def test_forces_only_steps_match_energy_steps_on_ladder(hip_device, use_graph, split):
+    """The forces-only pair kernel that steps use between energy evaluations gives the
+    forces of the energy variant bit for bit on a full-size batch (22 replicas of the
+    22,750-atom system, where every wave of the pair kernel shares its SIMD with
+    others): 10 steps with energies on the last step only equal 10 steps with energies
+    on every step."""
+    R = 22
+    s = systems.make_config2(frame=0)
+    params = systems.params_config3(R)
+    print(f"case ladder R={R} use_graph={use_graph} split={split}", flush=True)
+    xs = []
+    for every in (0, 1):
+        ctx = _native.NativeContext(s, R)
+        for r in range(R):
+            ctx.set_params(r, params[r])
+            ctx.set_positions(r, s.positions)
+            ctx.set_velocities(r, s.velocities)
+        ctx.set_tuning(**{**KERNELS["block_n3"], "pair_split": split}, use_graph=use_graph,
+                       sort_interval=20, energy_every_step=every)
+        ctx.step(10)
+        assert all(e["flags"] == 0 for e in ctx.energies())
+        xs.append(np.stack([ctx.get_positions(r) for r in range(R)]))
+        del ctx
+    assert np.array_equal(xs[0], xs[1]), float(np.max(np.abs(xs[0] - xs[1])))
 
 
 @pytest.mark.parametrize("field,value", [("pair_newton3", 2), ("use_graph", 2),
