@@ -130,10 +130,10 @@ typedef struct sdm_context sdm_context;
 typedef struct {
     int32_t pair_waves_per_group; /* 1, 2 or 4 wave64s per workgroup in the pair kernel */
     int32_t cell_cells_per_dim;   /* sort-grid resolution; 0 = automatic */
-    int32_t use_graph;            /* 1: replay steps from captured hipGraphs, in which the
-                                     bonded terms and the displaced-state chain run beside
-                                     the pair kernel; 0: launch each kernel directly, all
-                                     in order on the context's stream */
+    int32_t use_graph;            /* 1: replay steps from captured hipGraphs; 0: launch each
+                                     kernel directly.  Either way the bonded terms and the
+                                     displaced-state chain run beside the pair kernel on
+                                     two side streams. */
     int32_t pair_newton3;         /* block kernel only.  1: each pair once (half tile list,
                                      j-force scatter); 0: each ordered pair (no atomics) */
     int32_t pair_group;           /* 0: block kernel (one 64-atom i-block per wave, shared

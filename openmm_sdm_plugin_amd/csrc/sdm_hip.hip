@@ -404,31 +404,22 @@ int energy_slots(const sdm_context* c) {
     return c->NB * (split ? c->tune.pair_split : 1);
 }
 
-// evaluation of one step (everything before the Langevin update).  With `fork` (only
-// while a step graph is captured) the work forks after the slot order and the tile
-// lists: the pair kernel on the main stream, the bonded terms and the displaced-state
-// chain on two side streams; they join before the scalar stage.  Without it everything
-// runs in order on the main stream.  Direct launches do not fork: forked by events
-// between streams, they gave wrong trajectories for a few percent of contexts (side
-// kernels read main-stream data from before the fork), depending on how the process's
-// streams were spread over the hardware queues, while captured graphs, whose
-// dependencies are graph edges, and single-stream launches were exact in every check.
+// evaluation of one step (everything before the Langevin update).  After the slot order
+// and the tile lists the work forks: the pair kernel on the main stream, the bonded terms
+// and the displaced-state chain on two side streams; they join before the scalar stage.
 // (Forking before the tile lists, which the side work does not need, was measured
 // slower on 22 replicas, where the side kernels then slow the tile builder, which is on
 // the critical path, and no faster on one.)  pe0/pe1 (optional) are recorded around the
 // pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
-                  bool fork, hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
-    hipStream_t s = c->stream;
-    hipStream_t s1 = fork ? c->side[0] : s, s2 = fork ? c->side[1] : s;
+                  hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
+    hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
     enqueue_order(c, full);
     enqueue_tiles(c);
-    if (fork) {
-        (void)hipEventRecord(c->ev_fork, s);
-        (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
-        (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
-    }
+    (void)hipEventRecord(c->ev_fork, s);
+    (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
+    (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
     if (c->nbond + c->nang > 0)
         k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
                                                   c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
@@ -468,12 +459,10 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     energy = energy || c->tune.pair_group != 0 || !c->tune.pair_newton3;
     enqueue_pair(c, false, energy);
     if (pe1) (void)hipEventRecord(pe1, s);
-    if (fork) {
-        (void)hipEventRecord(c->ev_join[0], s1);
-        (void)hipEventRecord(c->ev_join[1], s2);
-        (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
-        (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
-    }
+    (void)hipEventRecord(c->ev_join[0], s1);
+    (void)hipEventRecord(c->ev_join[1], s2);
+    (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
+    (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     k_scalar<<<R, 256, 0, s>>>(energy_slots(c), c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
@@ -484,9 +473,9 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
                                                        c->dlig, c->rec, c->slot_of, c->F, c->flags);
 }
 
-void enqueue_step(sdm_context* c, bool full, bool energy, bool fork, hipEvent_t pe0 = nullptr,
+void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullptr,
                   hipEvent_t pe1 = nullptr) {
-    enqueue_eval(c, true, false, full, energy, fork, pe0, pe1);
+    enqueue_eval(c, true, false, full, energy, pe0, pe1);
     if (c->cm_every > 0)
         k_cm_remove<<<c->R, 1024, 0, c->stream>>>(c->N, c->Np, c->cm_every, c->mass, c->v64,
                                                    c->state);
@@ -509,7 +498,7 @@ int build_graph(sdm_context* c, bool full, bool energy) {
     hipGraph_t& graph = c->graph[full][energy];
     hipGraphExec_t& gexec = c->gexec[full][energy];
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, full, energy, true);
+    enqueue_step(c, full, energy);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e != hipSuccess) return fail(SDM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -1208,7 +1197,7 @@ int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
 
 int sdm_compute(sdm_context* c) {
     if (!c) return fail(SDM_ERR_INVALID, "null context");
-    enqueue_eval(c, false, false, take_order(c, false), true, false);
+    enqueue_eval(c, false, false, take_order(c, false), true);
     c->components_valid = false;
     return check_launch();
 }
@@ -1227,7 +1216,7 @@ int sdm_step(sdm_context* c, int32_t nsteps) {
             if (!c->gexec[full][energy]) CHK(build_graph(c, full, energy));
             HIPCHK(hipGraphLaunch(c->gexec[full][energy], c->stream));
         } else {
-            enqueue_step(c, full, energy, false);
+            enqueue_step(c, full, energy);
         }
     }
     return check_launch();
@@ -1266,7 +1255,7 @@ int sdm_compute_components(sdm_context* c) {
     }
     k_fill_ll<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->Fb_sep, n3);
     k_fill_d<<<(unsigned)((n3 + 255) / 256), 256, 0, c->stream>>>(c->dFd, n3);
-    enqueue_eval(c, false, true, take_order(c, false), true, false);
+    enqueue_eval(c, false, true, take_order(c, false), true);
     if (c->L > 0)
         k_scatter_dF<<<dim3(c->maxc + c->LB, c->R), 64, 0, c->stream>>>(
             c->N, c->Np, c->L, c->maxc, c->lig, c->cls, c->orig_s, c->cand, c->ncand, c->dB,
