@@ -1,7 +1,8 @@
 """Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py).
 
 CPU: the oracle still reproduces every fixture (a regression guard on the checker).
-GPU: the HIP path against the fixtures, at the north-star tolerances."""
+GPU: the HIP path against the fixtures, within 1e-5 relative (energies) and 1e-4 RMS
+relative (forces)."""
 import glob
 import os
 

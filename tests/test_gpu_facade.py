@@ -46,7 +46,7 @@ def test_cpp_facade_matches_c_abi(hip_device, tmp_path):
     res = json.loads(out.stdout.strip().splitlines()[-1])
     assert res["bound_error"] == "This Integrator is already bound to a context"
     assert res["softcore_error"] == "Unknown soft core method"
-    p = systems.params_config4()  # the driver's parameters
+    p = systems.params_config4()  # the parameters facade_driver.cpp sets
     p.lambda_ = 1.0
     p.seed = 4242
     ctx = _native.NativeContext(s, 1)

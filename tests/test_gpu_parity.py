@@ -1,6 +1,6 @@
 """GPU parity of the HIP SDM path against the FP64 oracle (called through the C ABI).
 
-Tolerances (north star, mixed FP32-compute / fixed-point mode):
+Tolerances (mixed FP32-compute / fixed-point mode):
   U0, U1, u, Ub, W, u'   within 1e-5 relative
   forces                 within 1e-4 RMS relative (per force component array)
 """
