@@ -812,7 +812,8 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 // ---------------------------------------------------------------------------
 
 // At least 5 waves per SIMD (at most 96 VGPRs, allocated in granules of 8): the energy
-// variant otherwise takes 97, which leaves room for 4.
+// variant otherwise takes 97, which leaves room for 4.  The forces-only variant takes
+// 84; held to 80 for 6 waves it spills 24 bytes per lane and runs 3.3% slower.
 template <int WPW, bool COUNT, bool ENERGY>
 __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_n3(
     int R, int N, int Np, int NB, int maxj, int split, const float4* __restrict__ arec,
