@@ -273,12 +273,18 @@ typedef struct {
     int32_t n_constraints;
     const int32_t* constraint_atoms;/* [2*n] */
     const double* constraint_length;/* nm */
-    double constraint_tol;          /* relative; 0 = 1e-10 */
+    double constraint_tol;          /* relative; 0 = keep the context's tolerance (1e-10
+                                       unless sdm_set_constraint_tolerance changed it) */
 } sdm_forcefield_terms;
 
 /* Replaces the context's further force-field terms (NULL removes them).  No counterpart
    call in the reference, whose DMS reader adds these forces to the System. */
 int sdm_set_forcefield_terms(sdm_context* ctx, const sdm_forcefield_terms* terms);
+
+/* Relative tolerance of the position constraints (|r|^2 within tol of d^2).  Replaces
+   Integrator::setConstraintTolerance, whose value the reference hands to the constraint
+   solver every step (ReferenceStochasticDynamicsSDM.cpp:252, getConstraintTolerance()). */
+int sdm_set_constraint_tolerance(sdm_context* ctx, double tol);
 
 /* Centre-of-mass motion removal every `frequency` steps (0 = never, the default): the
    mass-weighted mean velocity is subtracted from every atom with mass, before the
@@ -327,10 +333,15 @@ int sdm_clear_flags(sdm_context* ctx);
 int sdm_compute_components(sdm_context* ctx);
 int sdm_get_forces(sdm_context* ctx, int32_t replica, int32_t which, double* out);
 
-/* computeKineticEnergy (SDMKernels.h:77; ReferenceSDMKernels.cpp:108-135,345-347). */
+/* computeKineticEnergy (SDMKernels.h:77; ReferenceSDMKernels.cpp:108-135,345-347):
+   0.5 sum m v^2 with the velocities of constrained atoms projected onto the constraints
+   first (on a copy, as the reference's applyToVelocities call does). */
 int sdm_kinetic_energy(sdm_context* ctx, double* out /* [n_replicas] */);
 
-/* u' recorded every `every` steps (0 disables) into a per-replica ring of `capacity`. */
+/* u' recorded every `every` steps (0 disables) into a per-replica ring of `capacity`:
+   the u' evaluated on steps 0, every, 2 every, ... (the step index before the step), the
+   exchange record of a replica-exchange ladder.  sdm_get_u_history writes the newest
+   min(count held, capacity) records, oldest first, and their number to *count. */
 int sdm_set_u_history(sdm_context* ctx, int32_t every, int32_t capacity);
 int sdm_get_u_history(sdm_context* ctx, int32_t replica, double* out, int32_t capacity,
                       int32_t* count);

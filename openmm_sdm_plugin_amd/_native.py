@@ -120,7 +120,7 @@ EXPORTS = [
     "sdm_compute_components", "sdm_get_forces", "sdm_kinetic_energy", "sdm_set_u_history",
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
     "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
-    "sdm_set_forcefield_terms", "sdm_set_cm_motion_removal",
+    "sdm_set_forcefield_terms", "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance",
     "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
@@ -168,6 +168,7 @@ def load(path: str = LIB_PATH):
                                        C.POINTER(sdm_alignment_restraint), C.c_int32]
     lib.sdm_set_forcefield_terms.argtypes = [vp, C.POINTER(sdm_forcefield_terms)]
     lib.sdm_set_cm_motion_removal.argtypes = [vp, C.c_int32]
+    lib.sdm_set_constraint_tolerance.argtypes = [vp, C.c_double]
     for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
               "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
               "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
@@ -176,7 +177,7 @@ def load(path: str = LIB_PATH):
               "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
               "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms",
-              "sdm_set_cm_motion_removal"):
+              "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -334,6 +335,9 @@ class NativeContext:
 
     def set_cm_motion_removal(self, frequency):
         check(self._lib.sdm_set_cm_motion_removal(self.h, int(frequency)))
+
+    def set_constraint_tolerance(self, tol):
+        check(self._lib.sdm_set_constraint_tolerance(self.h, float(tol)))
 
     def set_forcefield_terms(self, ff, constraint_tol=0.0):
         """Torsions, impropers, positional restraints, 1-4 exceptions and constraints of a

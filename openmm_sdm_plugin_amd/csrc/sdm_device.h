@@ -35,6 +35,8 @@ struct DevState {
     // clock (not part of sdm_params)
     double time;
     int64_t step;
+    // u' records taken since the history was configured (sdm_set_u_history)
+    int64_t hist_n;
 };
 
 // Per-replica step record written by the scalar stage.

@@ -901,7 +901,7 @@ static int set_constraints(sdm_context* c, const sdm_forcefield_terms* t) {
     c->ncl = 0;
     drop_graph(c);
     if (nc == 0) return SDM_OK;
-    c->con_tol = t->constraint_tol > 0.0 ? t->constraint_tol : 1e-10;
+    if (t->constraint_tol > 0.0) c->con_tol = t->constraint_tol;
     CHK(dalloc(c, &c->cl_start, cl_start.size()));
     CHK(dalloc(c, &c->cl_atoms, cl_atoms.size()));
     CHK(dalloc(c, &c->con_start, con_start.size()));
@@ -1309,7 +1309,12 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
 
 int sdm_kinetic_energy(sdm_context* c, double* out) {
     if (!c || !out) return fail(SDM_ERR_INVALID, "null argument");
-    k_kinetic<<<c->R, 256, 0, c->stream>>>(c->N, c->Np, c->mass, c->v64, c->ke_out);
+    // the kinetic-energy tolerance of the reference (applyToVelocities(..., 1e-4),
+    // ReferenceSDMKernels.cpp:126) is a relative one; the projection here is iterated to
+    // 1e-12 of the atoms' speeds, well inside it
+    k_kinetic<<<c->R, 256, 0, c->stream>>>(c->N, c->Np, c->mass, c->invm, c->x64, c->v64,
+                                           c->ncl, c->cl_start, c->cl_atoms, c->con_start,
+                                           c->con_ij, c->cflag, 1e-12, c->flags, c->ke_out);
     CHK(check_launch());
     return download(c, out, c->ke_out, c->R);
 }
@@ -1317,22 +1322,46 @@ int sdm_kinetic_energy(sdm_context* c, double* out) {
 int sdm_set_u_history(sdm_context* c, int32_t every, int32_t capacity) {
     if (!c) return fail(SDM_ERR_INVALID, "null context");
     if (every < 0 || capacity <= 0) return fail(SDM_ERR_INVALID, "bad history settings");
-    if (capacity > c->hist_cap) CHK(dalloc(c, &c->uhist, (size_t)c->R * capacity));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (capacity > c->hist_cap) {
+        dfree(c, c->uhist);
+        c->uhist = nullptr;
+        CHK(dalloc(c, &c->uhist, (size_t)c->R * capacity));
+    }
     c->hist_every = every;
     c->hist_cap = capacity;
+    // the ring starts empty: no record of an earlier setting is returned
+    std::vector<DevState> st(c->R);
+    CHK(download(c, st.data(), c->state, c->R));
+    for (auto& x : st) x.hist_n = 0;
+    CHK(upload(c, c->state, st.data(), c->R));
     drop_graph(c);
     return SDM_OK;
 }
 
 int sdm_get_u_history(sdm_context* c, int32_t r, double* out, int32_t capacity, int32_t* count) {
     CHK(check_replica(c, r, false));
-    if (!out || !count) return fail(SDM_ERR_INVALID, "null argument");
-    int n = std::min(capacity, c->hist_cap);
-    CHK(download(c, out, c->uhist + (size_t)r * c->hist_cap, n));
+    if (!out || !count || capacity < 0) return fail(SDM_ERR_INVALID, "null argument");
+    std::vector<double> ring(c->hist_cap);
+    CHK(download(c, ring.data(), c->uhist + (size_t)r * c->hist_cap, ring.size()));
     DevState st;
     CHK(download(c, &st, c->state + r, 1));
-    int64_t rec_n = c->hist_every > 0 ? (st.step + c->hist_every - 1) / c->hist_every : 0;
-    *count = (int32_t)std::min<int64_t>(rec_n, n);
+    // record k (k = 0, 1, .. since the history was configured) sits in ring slot
+    // k % hist_cap; the ring holds the last min(rec_n, hist_cap) of them
+    int64_t rec_n = c->hist_every > 0 ? st.hist_n : 0;
+    int64_t held = std::min<int64_t>(rec_n, c->hist_cap);
+    int64_t n = std::min<int64_t>(held, capacity);
+    for (int64_t k = 0; k < n; k++) out[k] = ring[(size_t)((rec_n - n + k) % c->hist_cap)];
+    *count = (int32_t)n;
+    return SDM_OK;
+}
+
+int sdm_set_constraint_tolerance(sdm_context* c, double tol) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (!(tol > 0.0) || !std::isfinite(tol)) return fail(SDM_ERR_INVALID, "tolerance must be positive");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->con_tol = tol;
+    drop_graph(c);  // the tolerance is a kernel argument of the captured steps
     return SDM_OK;
 }
 
@@ -1454,6 +1483,10 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // the trials' steps take no u' records (the ring keeps the caller's); the state
+    // restored after each trial brings back the record count
+    const int hist_every = c->hist_every;
+    c->hist_every = 0;
     // one trial: n steps from the saved state after two unmeasured ones (the first
     // sorts and, with use_graph, captures), then the saved state back
     auto trial = [&](const sdm_tuning& t, int n, double& ms_out) -> int {
@@ -1515,6 +1548,8 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    c->hist_every = hist_every;
+    drop_graph(c);  // graphs captured during the trials record no history
     if (rc != SDM_OK) return rc;
     CHK(sdm_set_tuning(c, &best));
     if (chosen) *chosen = best;

@@ -2484,8 +2484,10 @@ __global__ __launch_bounds__(256) void k_scalar(
     Rr.noisescale = sqrt(kT * (1.0 - vscale * vscale));
     Rr.dt = stepSize;
     Rr.noise_step = S.step;
-    if (hist_every > 0 && advance && (S.step % hist_every) == 0)
-        uhist[(size_t)r * hist_cap + (size_t)((S.step / hist_every) % hist_cap)] = be;
+    if (hist_every > 0 && advance && (S.step % hist_every) == 0) {
+        uhist[(size_t)r * hist_cap + (size_t)(S.hist_n % hist_cap)] = be;
+        S.hist_n += 1;
+    }
     if (advance) {
         S.time += stepSize;
         S.step += 1;
@@ -2574,15 +2576,18 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
     }
 }
 
-// centre-of-mass motion removal (one workgroup per replica) on steps that are multiples of
-// `every`: v -= sum(m v) / sum(m) for every atom with mass
+// centre-of-mass motion removal (one workgroup per replica) on steps whose index (the
+// step count before the step, as OpenMM's CMMotionRemover tests it in the
+// updateContextState call that starts the step, LangevinIntegratorSDM.cpp:158) is a
+// multiple of `every`: v -= sum(m v) / sum(m) for every atom with mass.  Launched after
+// the scalar stage has advanced the count, so the step's index is state.step - 1.
 __global__ __launch_bounds__(1024) void k_cm_remove(int N, int Np, int every,
                                                     const double* __restrict__ mass,
                                                     double* __restrict__ v64,
                                                     const DevState* __restrict__ state) {
     __shared__ double red[4][1024];
     int r = blockIdx.x, t = threadIdx.x;
-    if (state[r].step % every != 0) return;  // uniform over the workgroup
+    if ((state[r].step - 1) % every != 0) return;  // uniform over the workgroup
     double* V = v64 + (size_t)r * 3 * Np;
     double p[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = t; i < N; i += 1024) {
@@ -2668,17 +2673,73 @@ __global__ __launch_bounds__(256) void k_constrain(
     }
 }
 
-__global__ __launch_bounds__(256) void k_kinetic(int N, int Np, const double* __restrict__ mass,
-                                                 const double* __restrict__ v64,
-                                                 double* __restrict__ out) {
+// Kinetic energy 0.5 sum m v^2 of every replica (one workgroup per replica), with the
+// velocities of constrained atoms first projected onto the constraints, as
+// ReferenceSDMKernels.cpp:108-135 does (applyToVelocities on a copy; the stored
+// velocities are not changed): per cluster, each constraint in turn removes the relative
+// velocity of its two atoms along their current bond vector in inverse proportion to
+// their masses (momentum is conserved), repeated until |r . dv| <= tol |r| (|v_i|+|v_j|)
+// for every constraint.  The converged result is the mass-weighted projection onto
+// r_c . (v_i - v_j) = 0, whatever solver reaches it.  Unconstrained atoms (cflag 0, or no
+// clusters) are summed as they are.  Sets flag 8 if a cluster does not converge.
+constexpr int KE_MAX_ITER = 500;
+__global__ __launch_bounds__(256) void k_kinetic(
+    int N, int Np, const double* __restrict__ mass, const double* __restrict__ invm,
+    const double* __restrict__ x64, const double* __restrict__ v64, int ncl,
+    const int* __restrict__ cl_start, const int* __restrict__ cl_atoms,
+    const int* __restrict__ con_start, const int* __restrict__ con_ij,
+    const int* __restrict__ cflag, double tol, int* __restrict__ flags,
+    double* __restrict__ out) {
     __shared__ double red[256];
     int r = blockIdx.x, t = threadIdx.x;
     const double* V = v64 + (size_t)r * 3 * Np;
+    const double* X = x64 + (size_t)r * 3 * Np;
     double e = 0.0;
     for (int i = t; i < N; i += 256) {
         double m = mass[i];
-        if (m > 0) e += m * (V[i] * V[i] + V[Np + i] * V[Np + i] + V[2 * Np + i] * V[2 * Np + i]);
+        if (m > 0 && !(ncl > 0 && cflag[i]))
+            e += m * (V[i] * V[i] + V[Np + i] * V[Np + i] + V[2 * Np + i] * V[2 * Np + i]);
     }
+    bool bad = false;
+    for (int q = t; q < ncl; q += 256) {
+        int a0 = cl_start[q], na = cl_start[q + 1] - a0;
+        int c0 = con_start[q], nc = con_start[q + 1] - c0;
+        double xv[SHAKE_MAX_ATOMS][3], vv[SHAKE_MAX_ATOMS][3], im[SHAKE_MAX_ATOMS];
+        for (int k = 0; k < na; k++) {
+            int a = cl_atoms[a0 + k];
+            im[k] = invm[a];
+            for (int d = 0; d < 3; d++) {
+                xv[k][d] = X[d * Np + a];
+                vv[k][d] = V[d * Np + a];
+            }
+        }
+        bool done = false;
+        for (int it = 0; it < KE_MAX_ITER && !done; it++) {
+            done = true;
+            for (int k = 0; k < nc; k++) {
+                int i = con_ij[2 * (c0 + k)], j = con_ij[2 * (c0 + k) + 1];
+                double rr[3], dv[3];
+                for (int d = 0; d < 3; d++) {
+                    rr[d] = xv[i][d] - xv[j][d];
+                    dv[d] = vv[i][d] - vv[j][d];
+                }
+                double r2 = dot3(rr, rr), rv = dot3(rr, dv);
+                double vs = sqrt(dot3(vv[i], vv[i])) + sqrt(dot3(vv[j], vv[j]));
+                if (fabs(rv) > tol * sqrt(r2) * vs) done = false;
+                double g = rv / (r2 * (im[i] + im[j]));
+                for (int d = 0; d < 3; d++) {
+                    vv[i][d] -= g * im[i] * rr[d];
+                    vv[j][d] += g * im[j] * rr[d];
+                }
+            }
+        }
+        bad = bad || !done;
+        for (int k = 0; k < na; k++) {
+            double m = mass[cl_atoms[a0 + k]];
+            e += m * dot3(vv[k], vv[k]);
+        }
+    }
+    if (bad) atomicOr(&flags[r], 8);
     red[t] = e;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {

@@ -94,7 +94,13 @@ class LangevinIntegratorSDM:
     def getStepSize(self): return self._p.step_size
     def setStepSize(self, v): self._set("step_size", float(v))
     def getConstraintTolerance(self): return self._constraint_tol
-    def setConstraintTolerance(self, v): self._constraint_tol = float(v)
+
+    def setConstraintTolerance(self, v):
+        """Integrator::setConstraintTolerance: the relative tolerance the constraint
+        solver is given every step (ReferenceStochasticDynamicsSDM.cpp:252)."""
+        self._constraint_tol = float(v)
+        if self._ctx is not None:
+            self._ctx._native.set_constraint_tolerance(self._constraint_tol)
     def getRandomNumberSeed(self): return int(self._p.seed)
     def setRandomNumberSeed(self, s): self._set("seed", int(s) & 0xFFFFFFFFFFFFFFFF)
     def getBindE(self): return self._bind_e
@@ -201,6 +207,7 @@ class Context:
             displ = np.asarray(system.displacement, dtype=np.float64)
             integrator._displ = displ.copy()
         self._native = _native.NativeContext(system, 1, displacement=displ)
+        self._native.set_constraint_tolerance(integrator.getConstraintTolerance())
         integrator._sync_to_device()
         self.setPositions(system.positions)
         self.setVelocities(system.velocities)

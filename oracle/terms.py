@@ -146,3 +146,54 @@ def constrain(x_old, x_unc, mass, pairs, lengths, tol=1e-14, max_iter=50):
         x[atoms] = _constrain_cluster(x_old[atoms], x[atoms], im[atoms], lp, d2[ks], tol,
                                       max_iter)
     return x
+
+
+def project_velocities(x, v, mass, pairs):
+    """Velocities with the component along every constraint removed: the kinetic energy of
+    ReferenceSDMKernels.cpp:108-135 is summed after applyToVelocities(positions, v, 1/m),
+    which makes the relative velocity of each constrained pair perpendicular to its bond.
+    Here solved exactly, cluster by cluster, as the mass-weighted projection
+        v' = v - M^-1 A^T lam,   (A M^-1 A^T) lam = A v,
+    with A the constraint rows (r_c on atom i, -r_c on atom j, r_c = x_i - x_j at the
+    current positions).  Returns v' (the input is not changed)."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.array(v, dtype=np.float64).reshape(-1, 3)
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    im = 1.0 / np.asarray(mass, dtype=np.float64)
+    parent = {}
+
+    def find(a):
+        while parent.setdefault(a, a) != a:
+            a = parent[a]
+        return a
+    for i, j in pairs:
+        parent[find(int(i))] = find(int(j))
+    groups = {}
+    for k, (i, _j) in enumerate(pairs):
+        groups.setdefault(find(int(i)), []).append(k)
+    for ks in groups.values():
+        atoms = sorted(set(pairs[ks].reshape(-1).tolist()))
+        loc = {a: n for n, a in enumerate(atoms)}
+        na, nc = len(atoms), len(ks)
+        A = np.zeros((nc, 3 * na))
+        for row, k in enumerate(ks):
+            i, j = int(pairs[k, 0]), int(pairs[k, 1])
+            rc = x[i] - x[j]
+            A[row, 3 * loc[i]:3 * loc[i] + 3] = rc
+            A[row, 3 * loc[j]:3 * loc[j] + 3] = -rc
+        minv = np.repeat(im[atoms], 3)
+        vc = v[atoms].reshape(-1)
+        lam = np.linalg.solve((A * minv) @ A.T, A @ vc)
+        v[atoms] = (vc - minv * (A.T @ lam)).reshape(-1, 3)
+    return v
+
+
+def kinetic_energy(x, v, mass, pairs=None):
+    """0.5 sum m v^2 over atoms with mass, after project_velocities when there are
+    constraints (ReferenceSDMKernels.cpp:108-135)."""
+    m = np.asarray(mass, dtype=np.float64)
+    if pairs is not None and len(pairs):
+        v = project_velocities(x, v, m, pairs)
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    keep = m > 0
+    return 0.5 * float(np.sum(m[keep, None] * v[keep] ** 2))

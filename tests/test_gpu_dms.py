@@ -107,3 +107,31 @@ def test_temoa_runs_with_constraints(hip_device):
         dof = 3 * s.n - c.shape[0]
         t = 2 * ke[r] / (dof * 0.0083144626)
         assert 250.0 < t < 350.0, t
+
+
+def test_temoa_kinetic_energy_projects_constraints(hip_device):
+    """computeKineticEnergy on the constrained system equals the oracle's: the velocities
+    projected onto the 20,278 constraints at the current positions (ReferenceSDMKernels.cpp
+    :108-135), for the file's velocities after 20 steps and for random velocities, whose
+    bond components the projection removes (the stored velocities are unchanged)."""
+    s, ff = load_temoa()
+    ctx = _native.NativeContext(s, 2)
+    for r in range(2):
+        ctx.set_params(r, temoa_params())
+        ctx.set_positions(r, s.positions)
+        ctx.set_velocities(r, s.velocities)
+    ctx.step(20)
+    rng = np.random.default_rng(11)
+    v_rand = rng.normal(scale=0.5, size=(s.n, 3))
+    ctx.set_velocities(1, v_rand)
+    ke = ctx.kinetic_energy()
+    m = np.asarray(s.mass)
+    for r in range(2):
+        x, v = ctx.get_positions(r), ctx.get_velocities(r)
+        want = OT.kinetic_energy(x, v, m, ff.constraints)
+        raw = OT.kinetic_energy(x, v, m)
+        assert abs(ke[r] - want) <= 1e-9 * want, (r, ke[r], want, raw)
+    assert np.array_equal(ctx.get_velocities(1), v_rand)
+    # the random velocities carry much kinetic energy along the bonds
+    assert OT.kinetic_energy(x, v_rand, m) > 1.1 * want
+    assert all(e["flags"] == 0 for e in ctx.energies())

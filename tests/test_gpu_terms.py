@@ -158,3 +158,32 @@ def test_cm_motion_removal(hip_device):
     scale = float((m[:, None] * np.abs(v)).sum())
     assert np.allclose(out[0], (m[:, None] * v).sum(0), atol=1e-6 * scale)
     assert np.max(np.abs(out[1])) < 1e-9 * scale
+
+
+def test_cm_motion_removal_interval(hip_device):
+    """With removal every 3 steps, the drift is taken out on the steps whose index (the
+    step count before the step) is 0, 3, 6, ... as OpenMM's CMMotionRemover tests it at
+    the start of the step: the drift is re-injected before each of 8 single steps and the
+    momentum after the step is zero exactly on those."""
+    s = systems.make_config1(frame=1)
+    p = systems.params_config1()
+    p.temperature, p.friction = 0.0, 0.0
+    m = np.asarray(s.mass)
+    drift = np.array([0.7, -0.4, 0.2])
+    ctx = _native.NativeContext(s, 1)
+    ctx.set_cm_motion_removal(3)
+    ctx.set_params(0, p)
+    ctx.set_positions(0, s.positions)
+    scale = float(m.sum()) * float(np.abs(drift).max())
+    removed = []
+    for k in range(8):
+        v = ctx.get_velocities(0)
+        v = v - (m[:, None] * v).sum(0) / m.sum() + drift  # momentum M * drift
+        ctx.set_velocities(0, v)
+        ctx.step(1)
+        assert ctx.energies()[0]["step"] == k + 1
+        mom = (m[:, None] * ctx.get_velocities(0)).sum(0)
+        removed.append(bool(np.max(np.abs(mom)) < 1e-6 * scale))
+        if not removed[-1]:
+            assert np.allclose(mom, m.sum() * drift, atol=1e-6 * scale)
+    assert removed == [k % 3 == 0 for k in range(8)], removed

@@ -102,3 +102,63 @@ def test_constraint_water_momentum_and_lengths():
         assert np.linalg.norm(xc[i] - xc[j]) == pytest.approx(L, rel=1e-13)
     # internal constraint forces: the mass-weighted displacement (momentum) is unchanged
     assert np.allclose((m[:, None] * xc).sum(0), (m[:, None] * x_unc).sum(0), atol=1e-15)
+
+
+def _water_cluster(rng, n=3):
+    """n rigid waters (O-H, O-H, H-H constraints) at random orientations, 0.4 nm apart."""
+    x, pairs, m = [], [], []
+    canon = np.array([[0.0, 0, 0], [0.09572, 0, 0], [-0.02399, 0.09266, 0]])
+    for k in range(n):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        x.extend(list(canon @ q.T + np.array([0.4 * k, 0, 0])))
+        pairs += [[3 * k, 3 * k + 1], [3 * k, 3 * k + 2], [3 * k + 1, 3 * k + 2]]
+        m += [15.9994, 1.008, 1.008]
+    return np.array(x), np.array(pairs), np.array(m)
+
+
+def test_velocity_projection_removes_bond_components():
+    """After the projection every constrained pair's relative velocity is perpendicular
+    to its bond, the momentum of every cluster is unchanged (the constraint impulses are
+    internal), and a second projection changes nothing."""
+    rng = np.random.default_rng(7)
+    x, pairs, m = _water_cluster(rng)
+    v = rng.normal(size=x.shape)
+    vp = OT.project_velocities(x, v, m, pairs)
+    for i, j in pairs:
+        assert abs((x[i] - x[j]) @ (vp[i] - vp[j])) < 1e-13
+    for k in range(3):
+        sl = slice(3 * k, 3 * k + 3)
+        assert np.allclose((m[sl, None] * vp[sl]).sum(0), (m[sl, None] * v[sl]).sum(0),
+                           atol=1e-13)
+    assert np.allclose(OT.project_velocities(x, vp, m, pairs), vp, atol=1e-13)
+
+
+def test_velocity_projection_diatomic_closed_form():
+    """Diatomic: only the relative velocity along the bond is removed, split between the
+    atoms in inverse proportion to their masses (a mass-for-inverse-mass slip fails)."""
+    x = np.array([[0.0, 0, 0], [0.06, 0.08, 0.0]])
+    v = np.array([[1.0, -2.0, 0.5], [0.3, 0.4, -1.0]])
+    m = np.array([12.0, 1.0])
+    r = x[0] - x[1]
+    u = r / np.linalg.norm(r)
+    w = (v[0] - v[1]) @ u  # relative speed along the bond
+    mu0, mu1 = (1 / m[0]) / (1 / m[0] + 1 / m[1]), (1 / m[1]) / (1 / m[0] + 1 / m[1])
+    want = np.array([v[0] - mu0 * w * u, v[1] + mu1 * w * u])
+    assert np.allclose(OT.project_velocities(x, v, m, [[0, 1]]), want, atol=1e-15)
+
+
+def test_kinetic_energy_rigid_rotor():
+    """A rigid diatomic spinning about its centre with a bond-stretching velocity added:
+    the projected kinetic energy is the rotational one, 0.5 I w^2, and without constraints
+    the stretching energy is kept."""
+    m = np.array([3.0, 1.0])
+    d = 0.2
+    x = np.array([[-0.25 * d, 0, 0], [0.75 * d, 0, 0]])  # centre of mass at the origin
+    w = 5.0
+    v_rot = np.array([[0, w * x[0, 0], 0], [0, w * x[1, 0], 0]])
+    v = v_rot + np.array([[-0.7, 0, 0], [2.1, 0, 0]])  # zero-momentum stretch
+    inertia = float(np.sum(m * x[:, 0] ** 2))
+    assert OT.kinetic_energy(x, v, m, [[0, 1]]) == pytest.approx(0.5 * inertia * w * w,
+                                                                 rel=1e-13)
+    assert OT.kinetic_energy(x, v, m) == pytest.approx(
+        0.5 * float(np.sum(m[:, None] * v ** 2)), rel=1e-15)
