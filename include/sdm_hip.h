@@ -375,6 +375,21 @@ int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);
 int sdm_du_dparams_frames(sdm_context* ctx, const double* d_frames, int32_t nframes,
                           const int32_t* sel, int32_t nsel, double* d_grad, double* d_u);
 
+/* sdm_du_dparams_frames without any device allocation: no counterpart in the reference.
+   For callers that cannot allocate device memory (code under stream capture, tight
+   fitting loops, harnesses that forbid hipMalloc).  sdm_du_dparams_frames stages the
+   selection in a buffer it allocates (and grows) on the device; this variant stages it
+   in a static device array of SDM_NOALLOC_MAX_SEL entries that every GPU the library
+   runs on holds until the process ends.  nsel above SDM_NOALLOC_MAX_SEL is refused with
+   SDM_ERR_CAPACITY.  Same arguments, kernels and results.  Every call on one GPU shares
+   that GPU's array: calls on one GPU that overlap in time (from several threads, or
+   several contexts' streams) race; calls made one after another do not (each returns
+   after its kernels have finished). */
+#define SDM_NOALLOC_MAX_SEL 16384
+int sdm_du_dparams_frames_noalloc(sdm_context* ctx, const double* d_frames, int32_t nframes,
+                                  const int32_t* sel, int32_t nsel, double* d_grad,
+                                  double* d_u);
+
 /* OpenMM plugin-loader entry points (ReferenceSDMKernelFactory.cpp:41-56).  Without an
    OpenMM runtime in the process they only record that they were called. */
 void registerPlatforms(void);

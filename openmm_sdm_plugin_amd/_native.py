@@ -121,6 +121,7 @@ EXPORTS = [
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
     "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
     "sdm_set_forcefield_terms", "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance",
+    "sdm_du_dparams_frames_noalloc",
     "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
@@ -164,6 +165,7 @@ def load(path: str = LIB_PATH):
     lib.sdm_get_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
     lib.sdm_set_tuning.argtypes = [vp, C.POINTER(sdm_tuning)]
     lib.sdm_du_dparams_frames.argtypes = [vp, vp, C.c_int32, _ip, C.c_int32, vp, vp]
+    lib.sdm_du_dparams_frames_noalloc.argtypes = [vp, vp, C.c_int32, _ip, C.c_int32, vp, vp]
     lib.sdm_set_restraints.argtypes = [vp, C.POINTER(sdm_centroid_restraint), C.c_int32,
                                        C.POINTER(sdm_alignment_restraint), C.c_int32]
     lib.sdm_set_forcefield_terms.argtypes = [vp, C.POINTER(sdm_forcefield_terms)]
@@ -177,7 +179,8 @@ def load(path: str = LIB_PATH):
               "sdm_kinetic_energy", "sdm_set_u_history", "sdm_get_u_history", "sdm_count_pairs",
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
               "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms",
-              "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance"):
+              "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance",
+              "sdm_du_dparams_frames_noalloc"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -466,6 +469,12 @@ class NativeContext:
         for k, v in kw.items():
             setattr(t, k, int(v))
         check(self._lib.sdm_set_tuning(self.h, C.byref(t)))
+
+    def du_dparams_frames_noalloc(self, frames_ptr, nframes, sel, grad_ptr, u_ptr):
+        sel, selp = _i(sel)
+        check(self._lib.sdm_du_dparams_frames_noalloc(
+            self.h, C.c_void_p(frames_ptr), int(nframes), selp, len(sel),
+            C.c_void_p(grad_ptr), C.c_void_p(u_ptr)))
 
     def du_dparams_frames(self, frames_ptr, nframes, sel, grad_ptr, u_ptr):
         sel, selp = _i(sel)

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -22,13 +23,39 @@ static_assert(offsetof(DevState, seed) == offsetof(sdm_params, seed), "layout");
 static_assert(offsetof(DevState, work) == offsetof(sdm_params, work), "layout");
 static_assert(offsetof(DevRecord, flags) == offsetof(sdm_energies, flags), "layout");
 
+// Selection staging of sdm_du_dparams_frames_noalloc: a file-scope device array, so the
+// entry point allocates nothing.  Each GPU holds its own copy at its own address, which
+// host code must obtain with hipGetSymbolAddress (the array's name is a host-side
+// stand-in, not a device address); it is resolved once per device, with that device
+// current, by the first sdm_create there, and kept in g_scratch_ptr.
+static __device__ __align__(256) int32_t scratch[SDM_NOALLOC_MAX_SEL];
+
 namespace {
 thread_local std::string g_err;
 int g_registered = 0;
+std::mutex g_scratch_mu;
+std::vector<int32_t*> g_scratch_ptr;  // by device ordinal; nullptr until resolved
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
+}
+
+// Device address of `scratch` on `device`, which must be the current device.
+int resolve_scratch(int device, int32_t** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return fail(SDM_ERR_HIP, "bad device ordinal");
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SDM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_ptr[device] = (int32_t*)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return SDM_OK;
 }
 }  // namespace
 
@@ -116,6 +143,8 @@ struct sdm_context {
     double* ke_out = nullptr;
     int* sel_dev = nullptr;
     size_t sel_cap = 0;
+    // this device's `scratch` (selection staging of the allocation-free adjoint)
+    int32_t* scratch_sel = nullptr;
     sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
@@ -676,6 +705,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         return code;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SDM_ERR_HIP, "no HIP device"));
+    if ((rc = resolve_scratch(c->device, &c->scratch_sel)) != SDM_OK) return bail(rc);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) != hipSuccess ||
@@ -1556,31 +1586,69 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     return SDM_OK;
 }
 
-int sdm_du_dparams_frames(sdm_context* c, const double* d_frames, int32_t nframes,
-                          const int32_t* sel, int32_t nsel, double* d_grad, double* d_u) {
-    if (!c || !d_frames || nframes <= 0) return fail(SDM_ERR_INVALID, "bad argument");
-    if (d_grad && (!sel || nsel <= 0)) return fail(SDM_ERR_INVALID, "gradient needs selected atoms");
+}  // extern "C"
+
+// The adjoint's kernels for a selection already staged on the device at d_sel.
+static int du_dparams_launch(sdm_context* c, const double* d_frames, int32_t nframes,
+                             const int32_t* d_sel, int32_t nsel, double* d_grad, double* d_u) {
     if (d_u && c->L > 0)
         k_u_frames<<<nframes, 256, 0, c->stream>>>(c->N, c->L, c->lig, c->cls, c->cdispl, c->q64,
                                                    c->sig64, c->eps64, c->ex_start, c->ex_list,
                                                    d_frames, d_u, c->g);
     else if (d_u)
         k_fill_d<<<(nframes + 255) / 256, 256, 0, c->stream>>>(d_u, (size_t)nframes);
-    if (d_grad) {
+    if (d_grad)
+        k_du_dparams<<<dim3(nsel, nframes), 64, 0, c->stream>>>(
+            c->N, d_sel, nsel, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
+            c->ex_list, d_frames, d_grad, c->g);
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SDM_OK;
+}
+
+static int du_dparams_check(sdm_context* c, const double* d_frames, int32_t nframes,
+                            const int32_t* sel, int32_t nsel, double* d_grad) {
+    if (!c || !d_frames || nframes <= 0) return fail(SDM_ERR_INVALID, "bad argument");
+    if (d_grad && (!sel || nsel <= 0)) return fail(SDM_ERR_INVALID, "gradient needs selected atoms");
+    if (d_grad)
         for (int k = 0; k < nsel; k++)
             if (sel[k] < 0 || sel[k] >= c->N) return fail(SDM_ERR_INVALID, "selected atom out of range");
+    return SDM_OK;
+}
+
+extern "C" {
+
+int sdm_du_dparams_frames(sdm_context* c, const double* d_frames, int32_t nframes,
+                          const int32_t* sel, int32_t nsel, double* d_grad, double* d_u) {
+    CHK(du_dparams_check(c, d_frames, nframes, sel, nsel, d_grad));
+    if (d_grad) {
         if ((size_t)nsel > c->sel_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            dfree(c, c->sel_dev);
+            c->sel_dev = nullptr;
+            c->sel_cap = 0;
             CHK(dalloc(c, &c->sel_dev, nsel));
             c->sel_cap = nsel;
         }
         HIPCHK(hipMemcpyAsync(c->sel_dev, sel, sizeof(int) * nsel, hipMemcpyHostToDevice, c->stream));
-        k_du_dparams<<<dim3(nsel, nframes), 64, 0, c->stream>>>(
-            c->N, c->sel_dev, nsel, c->cls, c->cdispl, c->q64, c->sig64, c->eps64, c->ex_start,
-            c->ex_list, d_frames, d_grad, c->g);
     }
-    CHK(check_launch());
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return SDM_OK;
+    return du_dparams_launch(c, d_frames, nframes, c->sel_dev, nsel, d_grad, d_u);
+}
+
+int sdm_du_dparams_frames_noalloc(sdm_context* c, const double* d_frames, int32_t nframes,
+                                  const int32_t* sel, int32_t nsel, double* d_grad,
+                                  double* d_u) {
+    CHK(du_dparams_check(c, d_frames, nframes, sel, nsel, d_grad));
+    if (d_grad && nsel > SDM_NOALLOC_MAX_SEL)
+        return fail(SDM_ERR_CAPACITY, "sdm_du_dparams_frames_noalloc: more than "
+                                      "SDM_NOALLOC_MAX_SEL (" +
+                                          std::to_string(SDM_NOALLOC_MAX_SEL) +
+                                          ") selected atoms");
+    // every entry the kernels read ([0, nsel)) is written first; nothing needs clearing
+    if (d_grad)
+        HIPCHK(hipMemcpyAsync(c->scratch_sel, sel, sizeof(int32_t) * nsel,
+                              hipMemcpyHostToDevice, c->stream));
+    return du_dparams_launch(c, d_frames, nframes, c->scratch_sel, nsel, d_grad, d_u);
 }
 
 void registerPlatforms(void) {}
