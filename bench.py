@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=22)
     ap.add_argument("--single-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 8-replica 99,920-atom ladder")
@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
+    ap.add_argument("--profile-large", type=int, default=0,
+                    help="profiling mode: N eager steps of the 8-replica 99,920-atom ladder "
+                         "(its benchmark tuning, kernels launched one by one), no other legs")
     ap.add_argument("--profile-pair", type=int, default=0,
                     help="profiling mode: the benchmark's autotuning and warm-up, then the "
                          "pair kernel alone N times (the launches the roofline figure "
@@ -69,27 +72,91 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(n_steps):
-    """FP64 Reference-form step (3 evaluations per step, single thread) on one replica."""
+def cpu_info():
+    """Core count (nproc) and model name (lscpu) of the machine the CPU legs run on."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "model": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(n_steps, warmup=10, runs=3, omp_steps=2):
+    """The FP64 oracle's Reference-platform step (E at x, E at x+d, bonded, execute,
+    Langevin) on the 22,750-atom replica: a warm-up, then the median of `runs` timed runs
+    of n_steps on one thread.  22 sequential replicas on one core give the same aggregate
+    steps/s.  Beside it, the one-replica-per-core form: as many replicas as threads (the
+    machine's cores, at most 16), each stepped omp_steps times after one warm-up step."""
     import numpy as np
     from openmm_sdm_plugin_amd import systems
     from oracle import oracle as O
     s = systems.make_config2(frame=0)
-    p = systems.params_config2()
+    params = systems.params_config3(22)
+    p = params[11]
     topo = O.Topology(s)
     op = O.make_params(p)
     x, v = s.positions.copy(), s.velocities.copy()
-    x, v, t, st, _ = O.ref_step(topo, op, x, v, 0.0, 0, p.seed, 0, 1)  # warm-up
-    t0 = time.perf_counter()
-    x, v, t, st, _ = O.ref_step(topo, op, x, v, t, st, p.seed, 0, n_steps)
-    dt = time.perf_counter() - t0
+    x, v, t, st, _ = O.ref_step(topo, op, x, v, 0.0, 0, p.seed, 0, warmup)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        x, v, t, st, _ = O.ref_step(topo, op, x, v, t, st, p.seed, 0, n_steps)
+        times.append(time.perf_counter() - t0)
     assert np.all(np.isfinite(x))
-    return {"value": n_steps / dt, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n_steps} steps (after 1 warm-up) of the 22,750-atom replica with the "
-                      "FP64 oracle's Reference-platform step (E at x, E at x+d, bonded, "
-                      "execute, Langevin), single thread; 22 sequential replicas give the "
-                      "same aggregate steps/s",
-            "seconds": dt}
+    med = float(np.median(times))
+    info = cpu_info()
+    threads = max(1, min(16, int(info["nproc"] or 1)))
+    reps = [params[k % 22] for k in range(threads)]
+    ops = [O.make_params(q) for q in reps]
+    xs, vs = [s.positions] * threads, [s.velocities] * threads
+    seeds = [q.seed for q in reps]
+    xs, vs = O.ref_step_replicas(topo, ops, xs, vs, seeds, 1, threads)  # warm-up
+    t0 = time.perf_counter()
+    xs, vs = O.ref_step_replicas(topo, ops, xs, vs, seeds, omp_steps, threads)
+    omp_dt = time.perf_counter() - t0
+    return {"value": n_steps / med, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"median of {runs} runs of {n_steps} steps (after {warmup} warm-up "
+                      "steps) of one 22,750-atom ladder replica, FP64 oracle in its "
+                      "Reference-platform form (E at x, E at x+d, bonded, execute, "
+                      "Langevin), one thread; 22 sequential replicas give the same "
+                      "aggregate steps/s",
+            "run_seconds": times, "nproc": info["nproc"], "cpu_model": info["model"],
+            "openmp_one_replica_per_core": {
+                "value": threads * omp_steps / omp_dt, "unit": "aggregate steps/s",
+                "threads": threads, "replicas": threads,
+                "sample": f"{threads} replicas x {omp_steps} steps after 1 warm-up step, "
+                          "one replica per OpenMP thread"}}
+
+
+# the launch configurations sdm_autotune picks on MI355X for the 22-replica 22,750-atom
+# ladder and the 8-replica 99,920-atom ladder (used by the profiling modes, which do not
+# autotune)
+LADDER_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=4, pair_waves_per_group=1,
+                     cell_cells_per_dim=20, sort_interval=20, use_graph=0, energy_every_step=0)
+LARGE_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=2, pair_waves_per_group=1,
+                    cell_cells_per_dim=33, sort_interval=20, energy_every_step=0)
+
+
+def large_ladder_context(systems, _native):
+    """8 replicas of the 99,920-atom two-guest system (lambda 0.1-0.45), from one frame."""
+    s5 = systems.make_config5(frame=0)
+    p5 = systems.params_config5(8)
+    big = _native.NativeContext(s5, 8)
+    for r in range(8):
+        big.set_params(r, p5[r])
+        big.set_positions(r, s5.positions)
+        big.set_velocities(r, s5.velocities)
+    return big
 
 
 def main():
@@ -105,6 +172,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if a.profile_large:
+        big = large_ladder_context(systems, _native)
+        big.set_tuning(**LARGE_TUNING, use_graph=0)
+        big.step(a.profile_large)
+        e5 = big.energies()
+        print(json.dumps({"profile_large_steps": a.profile_large, "step": e5[0]["step"],
+                          "flags": [x["flags"] for x in e5], "tuning": big.get_tuning()}))
+        return
     R = a.replicas
     frames = [systems.make_config2(frame=k) for k in range(R)]
     params = systems.params_config3(R)
@@ -116,9 +191,13 @@ def main():
     ctx.set_u_history(100, 4096)
     if a.profile or a.profile_pair:
         if a.profile_pair:
-            # the same sequence as the benchmark up to its roofline timing
+            # the same sequence as the benchmark up to its roofline timing (with
+            # --no-autotune: the configuration the autotuner picks on MI355X, so a counter
+            # run does not profile the autotuner's trials)
             if not a.no_autotune:
                 ctx.autotune(40)
+            else:
+                ctx.set_tuning(**LADDER_TUNING)
             ctx.step(a.warmup)
             ms = ctx.time_pair_kernel(a.profile_pair)
             print(json.dumps({"pair_kernel_ms_events": ms, "tuning": ctx.get_tuning()}))
@@ -199,13 +278,7 @@ def main():
     # guests displaced by +-3 nm, rc 1.2 nm), all starting from one frame ----
     large = None
     if not a.no_large:
-        s5 = systems.make_config5(frame=0)
-        p5 = systems.params_config5(8)
-        big = _native.NativeContext(s5, 8)
-        for r in range(8):
-            big.set_params(r, p5[r])
-            big.set_positions(r, s5.positions)
-            big.set_velocities(r, s5.velocities)
+        big = large_ladder_context(systems, _native)
         tuning5 = big.autotune(10) if not a.no_autotune else big.get_tuning()
         big.step(a.warmup)
         big.energies()

@@ -69,9 +69,11 @@ typedef struct {
     const double* angle_theta0;
     const double* angle_k;
     const double* displacement; /* [3*n_atoms] state-2 displacement d_i */
-    double box[3];              /* rectangular periodic box */
-    int32_t periodic;           /* must be 1 (CutoffPeriodic) */
-    double cutoff;              /* nm; box >= 2*cutoff in every dimension */
+    double box[3];              /* rectangular periodic box (ignored if not periodic) */
+    int32_t periodic;           /* 1: CutoffPeriodic (minimum image); 0: CutoffNonPeriodic
+                                   (no images, as OpenMM's NonbondedForce, the reference's
+                                   first example, example/test.py:63) */
+    double cutoff;              /* nm; periodic: box >= 2*cutoff in every dimension */
     double eps_rf;              /* reaction-field dielectric */
 } sdm_system_desc;
 
@@ -148,8 +150,9 @@ typedef struct {
                                      pot_energy) on the last step of each call only, the
                                      step whose record sdm_get_energies returns (u, W, sp,
                                      the work and the forces are evaluated every step);
-                                     1: on every step.  Block Newton-3 kernel only; the
-                                     other pair kernels always evaluate them. */
+                                     1: on every step.  The pair energies are FP64 sums
+                                     over the FP64 positions (their own kernel); the
+                                     forces are FP32 pair arithmetic. */
     int32_t pair_split;           /* block Newton-3 kernel only: 1, 2, 4 (the default) or
                                      8 waves per i-block, each taking every pair_split-th
                                      tile, so that short tile lists do not leave the GPU
@@ -157,6 +160,10 @@ typedef struct {
                                      Results differ only in rounding: each wave's i forces
                                      and pair energies are summed on their own before they
                                      are added up. */
+    int32_t side_fork;            /* where the bonded terms and the displaced-state chain
+                                     leave the main stream: 0 after the tile lists (beside
+                                     the pair kernel), 1 after the slot order (beside the
+                                     tile-list build as well).  Same results. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -350,8 +357,8 @@ int sdm_get_u_history(sdm_context* ctx, int32_t replica, double* out, int32_t ca
    non-excluded pairs with r < cutoff at x (p0) and of pairs with d_i != d_j and
    r < cutoff at x+d (pdelta). */
 int sdm_count_pairs(sdm_context* ctx, int64_t* p0, int64_t* pdelta /* [n_replicas] */);
-/* Average duration (ms) of the nonbonded pair kernel as most steps run it (forces only
-   unless energy_every_step is set): nsteps launches back to back, each after the slot
+/* Average duration (ms) of the nonbonded pair kernel as the steps run it (forces only;
+   the pair energies have a kernel of their own): nsteps launches back to back, each after the slot
    order's refresh and tile lists and with nothing running beside it, timed with HIP
    events on the context's stream.  Positions do not advance; forces must be recomputed
    afterwards. */
@@ -360,7 +367,8 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 /* Launch-configuration search (project-written autotuner): times each candidate (pair
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
    for `steps` steps, re-times the three fastest for 3 x `steps` steps, each with and
-   without graph replay, and keeps the fastest.  The state the context had before the
+   without graph replay and with either fork point of the side work, and keeps the
+   fastest.  The state the context had before the
    call is restored. */
 int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
 int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);

@@ -74,7 +74,8 @@ class sdm_tuning(C.Structure):
     _fields_ = [("pair_waves_per_group", C.c_int32), ("cell_cells_per_dim", C.c_int32),
                 ("use_graph", C.c_int32), ("pair_newton3", C.c_int32),
                 ("pair_group", C.c_int32), ("sort_interval", C.c_int32),
-                ("energy_every_step", C.c_int32), ("pair_split", C.c_int32)]
+                ("energy_every_step", C.c_int32), ("pair_split", C.c_int32),
+                ("side_fork", C.c_int32)]
 
 
 class sdm_centroid_restraint(C.Structure):

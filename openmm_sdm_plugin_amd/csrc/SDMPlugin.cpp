@@ -209,7 +209,7 @@ void Context::create(const std::vector<double>& displacement) {
     d.angle_k = sys_.ang_k_.data();
     d.displacement = displacement.data();
     for (int k = 0; k < 3; k++) d.box[k] = sys_.box_[k];
-    d.periodic = 1;
+    d.periodic = sys_.periodic_ ? 1 : 0;
     d.cutoff = sys_.cutoff_;
     d.eps_rf = sys_.eps_rf_;
     check(sdm_create(&d, 1, &h_));
@@ -256,9 +256,15 @@ State Context::getState(bool positions, bool velocities, bool forces) {
 // ------------------------------------------------------------ HIP kernels ----
 void HIPIntegrateLangevinStepSDMKernel::initialize(const System&, const LangevinIntegratorSDM&) {}
 
-void HIPIntegrateLangevinStepSDMKernel::execute(Context& context, LangevinIntegratorSDM& integrator) {
+void HIPIntegrateLangevinStepSDMKernel::execute(Context& context, LangevinIntegratorSDM& integrator,
+                                                double, double, double) {
     check(sdm_set_params(context.handle(), 0, &integrator.raw()));
     check(sdm_step(context.handle(), 1));
+    sdm_energies e;
+    check(sdm_get_energies(context.handle(), &e));
+    integrator.setBindE(e.bind_e);       // ReferenceSDMKernels.cpp:285
+    integrator.setPotEnergy(e.pot_energy);  // :287
+    check(sdm_get_params(context.handle(), 0, &integrator.raw()));  // schedule and work
 }
 
 double HIPIntegrateLangevinStepSDMKernel::computeKineticEnergy(Context& context,

@@ -44,6 +44,10 @@ public:
     void addBond(int a, int b, double r0, double k);
     void addAngle(int a, int b, int c, double theta0, double k);
     void setPeriodicBoxVectors(const Vec3& box) { box_ = box; }
+    // OpenMM NonbondedForce::CutoffPeriodic (the default here) or CutoffNonPeriodic
+    enum NonbondedMethod { CutoffNonPeriodic = 1, CutoffPeriodic = 2 };
+    void setNonbondedMethod(NonbondedMethod m) { periodic_ = m == CutoffPeriodic; }
+    bool usesPeriodicBoundaryConditions() const { return periodic_; }
     void setCutoffDistance(double rc) { cutoff_ = rc; }
     void setReactionFieldDielectric(double e) { eps_rf_ = e; }
     double getCutoffDistance() const { return cutoff_; }
@@ -57,20 +61,24 @@ private:
     std::vector<double> bond_r0_, bond_k_, ang_t0_, ang_k_;
     Vec3 box_{{0, 0, 0}};
     double cutoff_ = 1.0, eps_rf_ = 78.3;
+    bool periodic_ = true;
 };
 
 class LangevinIntegratorSDM;
 class Context;
 
-// Abstract platform kernel of the reference (SDMKernels.h:47-113).  Save/Make/Restore
-// exist for interface compatibility: the HIP implementation evaluates both states in
-// one pass, so they only record that they were called.
+// Abstract platform kernel of the reference (SDMKernels.h:47-113), with its method names
+// and arguments.  execute() takes the three energies that the reference's step() obtains
+// from OpenMM's force evaluations (SDMKernels.h:67-68, LangevinIntegratorSDM.cpp:160-181);
+// Save/Make/Restore exist for interface compatibility.
 class IntegrateLangevinStepSDMKernel {
 public:
     static std::string Name() { return "IntegrateLangevinStepSDM"; }
     virtual ~IntegrateLangevinStepSDMKernel() = default;
     virtual void initialize(const System& system, const LangevinIntegratorSDM& integrator) = 0;
-    virtual void execute(Context& context, LangevinIntegratorSDM& integrator) = 0;
+    virtual void execute(Context& context, LangevinIntegratorSDM& integrator,
+                         double State1Energy, double State2Energy,
+                         double BondAndRestraintEnergy) = 0;
     virtual double computeKineticEnergy(Context& context, const LangevinIntegratorSDM& integrator) = 0;
     virtual void SaveState1(Context&, const LangevinIntegratorSDM&) {}
     virtual void SaveState2(Context&, const LangevinIntegratorSDM&) {}
@@ -234,10 +242,15 @@ private:
 };
 
 // HIP implementations of the two kernel interfaces (through include/sdm_hip.h).
+// HIPIntegrateLangevinStepSDMKernel::execute is one whole step: the dual-state pass
+// evaluates U0, U1 and the group-1 energy itself, so it needs no force evaluation
+// before it and does not read the three energy arguments (pass NaN); the energies it
+// finds are in the integrator's getPotEnergy()/getBindE() afterwards.
 class HIPIntegrateLangevinStepSDMKernel : public IntegrateLangevinStepSDMKernel {
 public:
     void initialize(const System& system, const LangevinIntegratorSDM& integrator) override;
-    void execute(Context& context, LangevinIntegratorSDM& integrator) override;
+    void execute(Context& context, LangevinIntegratorSDM& integrator, double State1Energy,
+                 double State2Energy, double BondAndRestraintEnergy) override;
     double computeKineticEnergy(Context& context, const LangevinIntegratorSDM& integrator) override;
 };
 

@@ -80,6 +80,10 @@ struct sdm_context {
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
+    // 0: CutoffNonPeriodic (no images; the sort grid spans reg_lo .. reg_hi, the union of
+    // the positions' bounding boxes over the replicas, widened by GRID_MARGIN)
+    int periodic = 1;
+    std::vector<double> reg_box;  // [R][6] per replica lo, hi (empty entries: hi < lo)
     int NC = 0, ncx = 0, ncy = 0, ncz = 0, NC_alloc = 0;
     int maxj = 0, maxc = 0, nbond = 0, nang = 0, nbw = 1, max_ex = 0;
     Geom g{};
@@ -87,6 +91,7 @@ struct sdm_context {
     // topology (shared by all replicas)
     float *qs = nullptr, *sh = nullptr, *se = nullptr;
     double *q64 = nullptr, *sig64 = nullptr, *eps64 = nullptr, *invm = nullptr, *mass = nullptr;
+    double4* par64 = nullptr;  // [Np] (q sqrt(k_e), sigma/2, 2 sqrt(eps), 0): FP64 energies
     int *ex_start = nullptr, *ex_list = nullptr, *cls = nullptr, *lig = nullptr, *cellrank = nullptr;
     // blocks of the displaced-atom list: entries lb_start[b] .. + lb_count[b] (<= 64),
     // all of one displacement class, so a block's bounding box covers one group of atoms
@@ -108,6 +113,7 @@ struct sdm_context {
     unsigned long long* tmask = nullptr;
     long long* F = nullptr;
     double *e_pair = nullptr, *e_bond = nullptr, *u_part = nullptr;
+    double* e64 = nullptr;  // [R][NB * E64_SPLIT] FP64 pair-energy slots
     // SDMUtils restraints (group 1): centroid and alignment terms, their group atoms and
     // normalized weights, and one energy slot per (replica, restraint)
     DevCentroid* rcen = nullptr;
@@ -145,7 +151,7 @@ struct sdm_context {
     size_t sel_cap = 0;
     // this device's `scratch` (selection staging of the allocation-free adjoint)
     int32_t* scratch_sel = nullptr;
-    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4};
+    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4, 0};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
     // captured steps: [with a full sort][with the pair energies]
@@ -251,12 +257,46 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
 }
 
 // cell grid + Hilbert rank of every cell (the slot order of the spatial sort).
+constexpr double GRID_MARGIN = 0.5;  // nm around the positions of a non-periodic system
+
+// extent of the sort grid: the box, or (non-periodic) the region around the positions
+void grid_region(const sdm_context* c, double lo[3], double hi[3]) {
+    if (c->periodic) {
+        lo[0] = lo[1] = lo[2] = 0.0;
+        hi[0] = c->g.Lx;
+        hi[1] = c->g.Ly;
+        hi[2] = c->g.Lz;
+        return;
+    }
+    for (int d = 0; d < 3; d++) {
+        lo[d] = 1e300;
+        hi[d] = -1e300;
+    }
+    for (int r = 0; r < c->R; r++) {
+        const double* b = &c->reg_box[6 * (size_t)r];
+        if (!(b[0] <= b[3])) continue;  // no positions yet
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], b[d]);
+            hi[d] = std::max(hi[d], b[3 + d]);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        if (!(lo[d] <= hi[d])) lo[d] = hi[d] = 0.0;
+        lo[d] -= GRID_MARGIN;
+        hi[d] += GRID_MARGIN;
+    }
+}
+
 int setup_cells(sdm_context* c, int per_dim) {
-    double L[3] = {c->g.Lx, c->g.Ly, c->g.Lz};
+    double lo[3], hi[3];
+    grid_region(c, lo, hi);
+    double L[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
     int n[3];
     for (int d = 0; d < 3; d++) {
         int v = per_dim > 0 ? per_dim : (int)std::floor(L[d] / 0.3);
         n[d] = std::max(1, std::min(v, 1024));
+        c->g.go[d] = lo[d];
+        c->g.gs[d] = n[d] / L[d];
     }
     int NC = n[0] * n[1] * n[2];
     int bits = 1;
@@ -272,6 +312,9 @@ int setup_cells(sdm_context* c, int per_dim) {
     std::vector<int> rank(NC);
     for (int k = 0; k < NC; k++) rank[code[k].second] = k;
     if (NC > c->NC_alloc) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (void* p : {(void*)c->cellrank, (void*)c->ccount, (void*)c->cstart, (void*)c->cfill})
+            dfree(c, p);
         CHK(dalloc(c, &c->cellrank, NC));
         CHK(dalloc(c, &c->ccount, (size_t)c->R * NC));
         CHK(dalloc(c, &c->cstart, (size_t)c->R * (NC + 1)));
@@ -317,9 +360,12 @@ void enqueue_order(sdm_context* c, bool full) {
 // work (fewer than 8 workgroups per CU)
 bool few_blocks(const sdm_context* c) { return (long)c->NB * c->R < 2048; }
 
-// tile lists of the block pair kernels (the grouped kernel finds its j-blocks itself)
-void enqueue_tiles(sdm_context* c) {
-    if (c->tune.pair_group != 0) return;
+// tile lists of the block pair kernels and of the FP64 energy kernel (the grouped
+// kernel finds its j-blocks itself: with it, lists are built for energy evaluations only,
+// as half lists)
+void enqueue_tiles(sdm_context* c, bool energy) {
+    if (c->tune.pair_group != 0 && !energy) return;
+    const int half = c->tune.pair_group != 0 ? 1 : c->tune.pair_newton3;
     // One workgroup per (replica, i-block).  Its time is the serial path through it, so
     // when the workgroups do not fill the GPU (fewer than 8 per CU) each gets 8 waves for
     // the candidate masks, otherwise 4 (measured: one replica of 22,750 atoms +6%, 22
@@ -327,7 +373,7 @@ void enqueue_tiles(sdm_context* c) {
 #define SDM_TILES(NW)                                                                         \
     k_tiles<NW><<<dim3(c->NB, c->R), 64 * NW, 0, c->stream>>>(                                \
         c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,       \
-        c->ntiles, c->list_r2, c->g, c->tune.pair_newton3, c->flags)
+        c->ntiles, c->list_r2, c->g, half, c->flags)
     if (few_blocks(c)) SDM_TILES(8);
     else SDM_TILES(4);
 #undef SDM_TILES
@@ -336,7 +382,18 @@ void enqueue_tiles(sdm_context* c) {
 // slot order, block boxes and tile lists
 void enqueue_sort(sdm_context* c, bool full) {
     enqueue_order(c, full);
-    enqueue_tiles(c);
+    enqueue_tiles(c, false);
+}
+
+// FP64 pair energies into e64 (needs the tile lists of this evaluation)
+void enqueue_energy64(sdm_context* c) {
+    const int half = c->tune.pair_group != 0 ? 1 : c->tune.pair_newton3;
+    unsigned waves = (unsigned)(c->R * c->NB * E64_SPLIT);
+    size_t lds = (size_t)std::max(1, c->max_ex) * 64 * sizeof(int);
+    k_energy64<<<waves, 64, lds, c->stream>>>(c->R, c->N, c->Np, c->NB, c->maxj, E64_SPLIT,
+                                              half, c->arec, c->x64, c->par64, c->orig_s,
+                                              c->slot_of, c->tiles, c->tmask, c->ntiles,
+                                              c->ex_start, c->ex_list, c->max_ex, c->e64, c->g);
 }
 
 // ring capacity (entries per group) of the grouped pair kernel
@@ -427,12 +484,6 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
 #undef SDM_LAUNCH_PAIR
 }
 
-// pair-energy slots per replica in e_pair: one per (i-block, wave of the block)
-int energy_slots(const sdm_context* c) {
-    bool split = c->tune.pair_group == 0 && c->tune.pair_newton3;
-    return c->NB * (split ? c->tune.pair_split : 1);
-}
-
 // evaluation of one step (everything before the Langevin update).  After the slot order
 // and the tile lists the work forks: the pair kernel on the main stream, the bonded terms
 // and the displaced-state chain on two side streams; they join before the scalar stage.
@@ -445,10 +496,13 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
     enqueue_order(c, full);
-    enqueue_tiles(c);
+    // the side work needs the slot order, block boxes and cleared accumulator, not the
+    // tile lists: it forks before or after their build (tuning.side_fork)
+    if (!c->tune.side_fork) enqueue_tiles(c, energy);
     (void)hipEventRecord(c->ev_fork, s);
     (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
     (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
+    if (c->tune.side_fork) enqueue_tiles(c, energy);
     if (c->nbond + c->nang > 0)
         k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
                                                   c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
@@ -469,7 +523,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
         k_ligcand<<<R, 64, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
                                     c->g, c->flags);
-        k_deltaA<<<dim3(c->maxc, c->LB, R), 64, 0, s2>>>(
+        k_deltaA<DELTA_WAVES><<<dim3(c->maxc, c->LB, R), 64 * DELTA_WAVES, 0, s2>>>(
             N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
@@ -478,21 +532,21 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64, c->ex_start,
             c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g, nullptr);
-        k_deltaB<<<dim3(c->maxc, R), 64, 0, s2>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
+        k_deltaB<DELTA_WAVES><<<dim3(c->maxc, R), 64 * DELTA_WAVES, 0, s2>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
                                                   c->sig64, c->eps64, c->ex_start, c->ex_list,
                                                   c->x64, c->orig_s, c->cand, c->ncand, c->maxc,
                                                   c->dB, c->g);
     }
     if (pe0) (void)hipEventRecord(pe0, s);
-    // without the Newton-3 block kernel the energies are always evaluated
-    energy = energy || c->tune.pair_group != 0 || !c->tune.pair_newton3;
-    enqueue_pair(c, false, energy);
+    // forces in FP32; the pair energies, when asked for, in FP64 by their own kernel
+    enqueue_pair(c, false, false);
     if (pe1) (void)hipEventRecord(pe1, s);
+    if (energy) enqueue_energy64(c);
     (void)hipEventRecord(c->ev_join[0], s1);
     (void)hipEventRecord(c->ev_join[1], s2);
     (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
     (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
-    k_scalar<<<R, 256, 0, s>>>(energy_slots(c), c->nbw, c->LB, c->e_pair, c->e_bond, c->u_part, c->state,
+    k_scalar<<<R, 256, 0, s>>>(c->NB * E64_SPLIT, c->nbw, c->LB, c->e64, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
                                c->ncen + c->nali, c->e_t1, c->e_t2, c->ntb);
@@ -578,10 +632,28 @@ sdm_params default_params() {
     return p;
 }
 
+// Capacity of one i-block's tile list (the j-blocks whose box lies within the list
+// radius of its box), so that tile storage grows as N rather than N^2.  At the mean
+// density a block of 64 atoms fills v_b = 64 V / N; a generous bound on its reach from its
+// centre is 1.5 (3 v_b / 4 pi)^(1/3) (Hilbert-ordered blocks are near-cubic), so candidate
+// centres lie within r_l + 2 of those of the i-block's, a sphere that holds NB V_s / V
+// blocks on average.  Three times that, for density fluctuations, at least 64 and at most
+// NB (a list that would exceed it sets SDM_FLAG_CAPACITY).  Non-periodic systems: NB.
+int tile_capacity(const sdm_system_desc* sys, int NB) {
+    if (!sys->periodic) return NB;
+    const double pi = 3.14159265358979323846;
+    double V = sys->box[0] * sys->box[1] * sys->box[2];
+    double vb = 64.0 * V / sys->n_atoms;
+    double reach = sys->cutoff + 1e-3 + 2.0 * 1.5 * std::cbrt(3.0 * vb / (4.0 * pi));
+    double nb = NB * (4.0 / 3.0) * pi * reach * reach * reach / V;
+    return std::max(1, std::min(NB, std::max(64, (int)std::ceil(3.0 * nb))));
+}
+
 int upload_nb_params(sdm_context* c, const double* q, const double* sig, const double* eps) {
     int N = c->N, Np = c->Np;
     std::vector<float> qs(Np, 0.f), sh(Np, 0.f), se(Np, 0.f);
     std::vector<double> q64(Np, 0.0), s64(Np, 0.0), e64(Np, 0.0);
+    std::vector<double4> p64(Np, double4{0.0, 0.0, 0.0, 0.0});
     double sk = std::sqrt(SDM_ONE_4PI_EPS0);
     for (int i = 0; i < N; i++) {
         if (!(sig[i] >= 0) || !(eps[i] >= 0) || !std::isfinite(q[i]))
@@ -592,6 +664,7 @@ int upload_nb_params(sdm_context* c, const double* q, const double* sig, const d
         q64[i] = q[i] * sk;
         s64[i] = sig[i];
         e64[i] = eps[i];
+        p64[i] = double4{q[i] * sk, 0.5 * sig[i], 2.0 * std::sqrt(eps[i]), 0.0};
     }
     CHK(upload(c, c->qs, qs.data(), Np));
     CHK(upload(c, c->sh, sh.data(), Np));
@@ -599,6 +672,7 @@ int upload_nb_params(sdm_context* c, const double* q, const double* sig, const d
     CHK(upload(c, c->q64, q64.data(), Np));
     CHK(upload(c, c->sig64, s64.data(), Np));
     CHK(upload(c, c->eps64, e64.data(), Np));
+    CHK(upload(c, c->par64, p64.data(), Np));
     return SDM_OK;
 }
 
@@ -619,11 +693,11 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     if (n_replicas <= 0) return fail(SDM_ERR_INVALID, "n_replicas must be positive");
     if (!sys->charge || !sys->sigma || !sys->epsilon || !sys->mass || !sys->displacement)
         return fail(SDM_ERR_INVALID, "per-atom arrays must not be null");
-    if (!sys->periodic) return fail(SDM_ERR_UNSUPPORTED, "only CutoffPeriodic systems are supported");
     if (!(sys->cutoff > 0)) return fail(SDM_ERR_INVALID, "cutoff must be positive");
-    for (int d = 0; d < 3; d++)
-        if (!(sys->box[d] >= 2.0 * sys->cutoff))
-            return fail(SDM_ERR_INVALID, "the cutoff distance cannot be greater than half the periodic box size");
+    if (sys->periodic)
+        for (int d = 0; d < 3; d++)
+            if (!(sys->box[d] >= 2.0 * sys->cutoff))
+                return fail(SDM_ERR_INVALID, "the cutoff distance cannot be greater than half the periodic box size");
     if (!(sys->eps_rf > 0)) return fail(SDM_ERR_INVALID, "eps_rf must be positive");
     for (int i = 0; i < N; i++)
         if (!(sys->mass[i] >= 0)) return fail(SDM_ERR_INVALID, "masses must be >= 0");
@@ -724,15 +798,27 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     c->nbond = sys->n_bonds;
     c->nang = sys->n_angles;
     c->nbw = std::max(1, (c->nbond + c->nang + 255) / 256);
-    c->maxj = c->NB;
+    c->maxj = tile_capacity(sys, c->NB);
     c->maxc = std::min(c->NB, 256 * std::max(1, c->ncls) + 64);
     Geom& g = c->g;
-    g.Lx = sys->box[0];
-    g.Ly = sys->box[1];
-    g.Lz = sys->box[2];
-    g.iLx = 1.0 / g.Lx;
-    g.iLy = 1.0 / g.Ly;
-    g.iLz = 1.0 / g.Lz;
+    c->periodic = sys->periodic ? 1 : 0;
+    c->reg_box.assign(6 * (size_t)n_replicas, 0.0);
+    for (int r = 0; r < n_replicas; r++) {  // no positions yet: empty boxes
+        c->reg_box[6 * (size_t)r] = 1.0;
+        c->reg_box[6 * (size_t)r + 3] = -1.0;
+    }
+    if (c->periodic) {
+        g.Lx = sys->box[0];
+        g.Ly = sys->box[1];
+        g.Lz = sys->box[2];
+        g.iLx = 1.0 / g.Lx;
+        g.iLy = 1.0 / g.Ly;
+        g.iLz = 1.0 / g.Lz;
+    } else {
+        // no images: L * round(d / L) with a zero inverse is 0 for every d
+        g.Lx = g.Ly = g.Lz = 1e30;
+        g.iLx = g.iLy = g.iLz = 0.0;
+    }
     g.bx = (float)g.Lx;
     g.by = (float)g.Ly;
     g.bz = (float)g.Lz;
@@ -747,11 +833,18 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     g.krff = (float)g.krf;
     g.crff = (float)g.crf;
     c->list_r2 = (float)((rcut + 1e-3) * (rcut + 1e-3));
+    if (c->periodic) {
+        g.oix = (float)(g.Lx - (rcut + 1e-3) - 1e-3);
+        g.oiy = (float)(g.Ly - (rcut + 1e-3) - 1e-3);
+        g.oiz = (float)(g.Lz - (rcut + 1e-3) - 1e-3);
+    } else {
+        g.oix = g.oiy = g.oiz = 1e30f;
+    }
     int N_ = N, Np = c->Np, R = c->R, NB = c->NB;
     size_t RNp = (size_t)R * Np;
 #define A(ptr, n) \
     if ((rc = dalloc(c, &c->ptr, (n))) != SDM_OK) return bail(rc)
-    A(qs, Np); A(sh, Np); A(se, Np); A(q64, Np); A(sig64, Np); A(eps64, Np);
+    A(qs, Np); A(sh, Np); A(se, Np); A(q64, Np); A(sig64, Np); A(eps64, Np); A(par64, Np);
     A(invm, Np); A(mass, Np); A(ex_start, N_ + 1); A(ex_list, elist.size());
     A(cls, Np); A(lig, std::max(1, c->L)); A(cdispl, cd.size());
     A(lb_start, lbs.size()); A(lb_count, lbn.size());
@@ -764,6 +857,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
     A(e_pair, (size_t)R * NB * 8); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
+    A(e64, (size_t)R * NB * E64_SPLIT);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
     A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
     A(dB, (size_t)R * c->maxc * 64); A(state, R); A(rec, R); A(flags, R);
@@ -1184,7 +1278,20 @@ static int get_vec(sdm_context* c, const double* base, int32_t r, double* xyz) {
 
 int sdm_set_positions(sdm_context* c, int32_t r, const double* xyz) {
     if (c) c->since_sort = 0;  // the slot order is re-sorted from the new positions
-    return set_vec(c, c->x64, r, xyz);
+    CHK(set_vec(c, c->x64, r, xyz));
+    if (c->periodic) return SDM_OK;
+    // non-periodic: the sort grid follows the positions
+    double* b = &c->reg_box[6 * (size_t)r];
+    for (int d = 0; d < 3; d++) {
+        b[d] = 1e300;
+        b[3 + d] = -1e300;
+    }
+    for (int i = 0; i < c->N; i++)
+        for (int d = 0; d < 3; d++) {
+            b[d] = std::min(b[d], xyz[3 * i + d]);
+            b[3 + d] = std::max(b[3 + d], xyz[3 * i + d]);
+        }
+    return setup_cells(c, c->tune.cell_cells_per_dim);
 }
 int sdm_get_positions(sdm_context* c, int32_t r, double* xyz) { return get_vec(c, c->x64, r, xyz); }
 int sdm_set_velocities(sdm_context* c, int32_t r, const double* xyz) { return set_vec(c, c->v64, r, xyz); }
@@ -1198,6 +1305,28 @@ __global__ void k_aos_to_soa(int N, int Np, const double* __restrict__ in, doubl
     int r = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     for (int d = 0; d < 3; d++) out[((size_t)r * 3 + d) * Np + i] = in[((size_t)r * N + i) * 3 + d];
+}
+// per replica: lo (3) and hi (3) of the FP64 positions (one workgroup per replica)
+__global__ __launch_bounds__(256) void k_extent(int N, int Np, const double* __restrict__ x64,
+                                                double* __restrict__ out) {
+    __shared__ double red[6][256];
+    int r = blockIdx.x, t = threadIdx.x;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int i = t; i < N; i += 256)
+        for (int d = 0; d < 3; d++) {
+            v[d] = fmin(v[d], X[d * Np + i]);
+            v[3 + d] = fmax(v[3 + d], X[d * Np + i]);
+        }
+    for (int k = 0; k < 6; k++) red[k][t] = v[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o)
+            for (int k = 0; k < 6; k++)
+                red[k][t] = k < 3 ? fmin(red[k][t], red[k][t + o]) : fmax(red[k][t], red[k][t + o]);
+        __syncthreads();
+    }
+    if (t < 6) out[(size_t)r * 6 + t] = red[t][0];
 }
 __global__ void k_soa_to_aos(int N, int Np, const double* __restrict__ in, double* __restrict__ out) {
     int r = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1214,7 +1343,16 @@ int sdm_set_positions_device(sdm_context* c, const double* d_xyz) {
     k_aos_to_soa<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, d_xyz, c->x64);
     CHK(check_launch());
     HIPCHK(hipStreamSynchronize(c->stream));
-    return SDM_OK;
+    if (c->periodic) return SDM_OK;
+    // non-periodic: the sort grid follows the positions (per replica bounding boxes)
+    double* ext = nullptr;
+    CHK(dalloc(c, &ext, 6 * (size_t)c->R));
+    k_extent<<<c->R, 256, 0, c->stream>>>(c->N, c->Np, c->x64, ext);
+    int rc = check_launch();
+    if (rc == SDM_OK) rc = download(c, c->reg_box.data(), ext, 6 * (size_t)c->R);
+    dfree(c, ext);
+    if (rc != SDM_OK) return rc;
+    return setup_cells(c, c->tune.cell_cells_per_dim);
 }
 
 int sdm_get_positions_device(sdm_context* c, double* d_xyz) {
@@ -1406,7 +1544,7 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
         k_ligcand<<<c->R, 64, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
                                       c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
                                       c->list_r2, c->g, c->flags);
-        k_deltaA<<<dim3(c->maxc, c->LB, c->R), 64, 0, s>>>(
+        k_deltaA<DELTA_WAVES><<<dim3(c->maxc, c->LB, c->R), 64 * DELTA_WAVES, 0, s>>>(
             c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
             c->q64, c->sig64, c->eps64,
             c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
@@ -1442,7 +1580,7 @@ int sdm_time_pair_kernel(sdm_context* c, int32_t nsteps, double* avg_ms) {
         for (int s = 0; s < nsteps; s++) {
             enqueue_sort(c, take_order(c, false));
             (void)hipEventRecord(ev[2 * s], c->stream);
-            enqueue_pair(c, false, c->tune.energy_every_step != 0);
+            enqueue_pair(c, false, false);
             (void)hipEventRecord(ev[2 * s + 1], c->stream);
         }
         rc = check_launch();
@@ -1487,6 +1625,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4 or 8");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
+    if (t->side_fork != 0 && t->side_fork != 1)
+        return fail(SDM_ERR_INVALID, "side_fork must be 0 or 1");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -1507,7 +1647,9 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     CHK(download(c, fl.data(), c->flags, c->R));
     sdm_tuning best = c->tune;
     double best_ms = 1e300;
-    double L = std::min(c->g.Lx, std::min(c->g.Ly, c->g.Lz));
+    double glo[3], ghi[3];
+    grid_region(c, glo, ghi);
+    double L = std::min(ghi[0] - glo[0], std::min(ghi[1] - glo[1], ghi[2] - glo[2]));
     const double cell_sides[] = {0.24, 0.3, 0.4};
     const int waves[] = {1, 2, 4};
     hipEvent_t e0, e1;
@@ -1558,7 +1700,8 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
             }
     // the three fastest again with trials three times as long (configurations within a
     // few percent of each other are not told apart reliably by short trials), each with
-    // the step replayed from a captured graph and launched kernel by kernel: which is
+    // the side work forked before or after the tile lists, and with the step replayed
+    // from a captured graph and launched kernel by kernel: which is
     // faster depends on the size of the step, and it changes the ranking of the other
     // settings (graph replay of the three-stream step was measured slower than direct
     // launches at 22 x 22,750 atoms, where 8 waves per i-block won under replay and 4
@@ -1566,9 +1709,10 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     std::stable_sort(timed.begin(), timed.end(),
                      [](const auto& p, const auto& q) { return p.first < q.first; });
     for (size_t k = 0; k < timed.size() && k < 3 && rc == SDM_OK; k++)
-        for (int graph = 0; graph < 2 && rc == SDM_OK; graph++) {
+        for (int variant = 0; variant < 4 && rc == SDM_OK; variant++) {
             sdm_tuning t = timed[k].second;
-            t.use_graph = graph;
+            t.use_graph = variant & 1;
+            t.side_fork = variant >> 1;
             double ms = 0.0;
             rc = trial(t, 3 * steps, ms);
             if (rc == SDM_OK && ms < best_ms) {

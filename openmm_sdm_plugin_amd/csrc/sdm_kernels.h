@@ -24,12 +24,30 @@
 
 namespace sdm {
 
+// Box and cutoff constants of every kernel.  A non-periodic system (CutoffNonPeriodic)
+// has a box of 1e30 with zero inverse: every image shift L * round(d / L) is then 0 and
+// coordinates are never wrapped.
 struct Geom {
     float bx, by, bz, ibx, iby, ibz;  // box and inverse (FP32 path)
     double Lx, Ly, Lz, iLx, iLy, iLz; // FP64 path
     float rc2f, krff, crff;
     double rc2, krf, crf;
+    // one periodic shift s for a whole tile (two blocks with centre difference dc and
+    // half-extents h_i, h_j) is exact for every pair within the list radius r_l when
+    // |dc - s| + h_i + h_j <= L - r_l in each dimension: a pair that s puts k != 0 box
+    // lengths off its minimum image then lies at least L - (L - r_l) = r_l away in truth,
+    // and at least L/2 >= r_c as evaluated, so it is left out either way.  oix..oiz hold
+    // L - r_l less a 1e-3 nm margin for FP32 rounding (1e30 if not periodic)
+    float oix, oiy, oiz;
+    // sort grid: cell c_d = floor((w_d - go_d) * gs_d), clamped to the grid, w the
+    // (wrapped, if periodic) coordinate.  Periodic: go = 0, gs = cells / L; otherwise
+    // the grid spans a region around the positions (atoms outside take edge cells)
+    double go[3], gs[3];
 };
+
+__device__ __forceinline__ int grid_cell(double w, double go, double gs, int n) {
+    return (int)fmin(fmax(floor((w - go) * gs), 0.0), (double)(n - 1));
+}
 
 // ---------------------------------------------------------------------------
 // spatial sort (deterministic): cell rank per atom, counting sort, per-cell sort by
@@ -46,9 +64,9 @@ __global__ void k_bin(int N, int Np, const double* __restrict__ x64, int* __rest
     wx -= g.Lx * floor(wx * g.iLx);
     wy -= g.Ly * floor(wy * g.iLy);
     wz -= g.Lz * floor(wz * g.iLz);
-    int cx = min(max((int)(wx * g.iLx * ncx), 0), ncx - 1);
-    int cy = min(max((int)(wy * g.iLy * ncy), 0), ncy - 1);
-    int cz = min(max((int)(wz * g.iLz * ncz), 0), ncz - 1);
+    int cx = grid_cell(wx, g.go[0], g.gs[0], ncx);
+    int cy = grid_cell(wy, g.go[1], g.gs[1], ncy);
+    int cz = grid_cell(wz, g.go[2], g.gs[2], ncz);
     int k = cellrank[(cx * ncy + cy) * ncz + cz];
     key[(size_t)r * Np + i] = k;
     atomicAdd(&ccount[(size_t)r * NC + k], 1);
@@ -285,8 +303,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
     auto mask_of = [&](int jb2) {
         int s = jb2 * 64 + lane;
         float4 p = P[s];
-        // one periodic shift for the whole tile when both blocks' boxes fit in half a
-        // box (as in the pair kernel); otherwise the minimum image per group
+        // one periodic shift for the whole tile when it is exact for every pair within
+        // the list radius (Geom::oix, as in the pair kernel); otherwise the minimum image
+        // per group
         float4 lj = lo[jb2], hj = hi[jb2];
         float dcx = 0.5f * (li.x + hi_i.x - lj.x - hj.x);
         float dcy = 0.5f * (li.y + hi_i.y - lj.y - hj.y);
@@ -294,9 +313,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
         float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
               sz = g.bz * rintf(dcz * g.ibz);
         bool one_image =
-            fabsf(dcx - sx) + 0.5f * (hi_i.x - li.x + hj.x - lj.x) < 0.5f * g.bx - 1e-3f &&
-            fabsf(dcy - sy) + 0.5f * (hi_i.y - li.y + hj.y - lj.y) < 0.5f * g.by - 1e-3f &&
-            fabsf(dcz - sz) + 0.5f * (hi_i.z - li.z + hj.z - lj.z) < 0.5f * g.bz - 1e-3f;
+            fabsf(dcx - sx) + 0.5f * (hi_i.x - li.x + hj.x - lj.x) < g.oix &&
+            fabsf(dcy - sy) + 0.5f * (hi_i.y - li.y + hj.y - lj.y) < g.oiy &&
+            fabsf(dcz - sz) + 0.5f * (hi_i.z - li.z + hj.z - lj.z) < g.oiz;
         if (one_image) {
             p.x += sx;
             p.y += sy;
@@ -401,9 +420,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
 //   NEWTON = true:  half tile list (j-block >= i-block, j > i inside the diagonal tile),
 //                   every pair once; the j forces of 8 j atoms at a time are summed over
 //                   the wave by a reduce-scatter and added with int64 atomics.
-// When a tile's two boxes fit in half a box length the periodic image is one shift for
-// the whole tile (applied to the i coordinates); otherwise each pair takes the minimum
-// image.  Pair energies are accumulated in FP64 per pair and the per-tile FP32 force and
+// When one shift is exact for every pair of a tile within the list radius (Geom::oix)
+// the periodic image is that shift for the whole tile (applied to the i coordinates);
+// otherwise each pair takes the minimum image.  Pair energies are accumulated in FP64 per pair and the per-tile FP32 force and
 // reaction-field sums in FP64 in tile order; fixed-point atomics commute, so results are
 // bit-reproducible.
 // ---------------------------------------------------------------------------
@@ -734,7 +753,8 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
             if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
         }
         if (jb == ib) em |= NEWTON ? ((2ull << lane) - 1ull) : (1ull << lane);
-        // one periodic shift for the whole tile when both boxes fit in half a box
+        // one periodic shift for the whole tile when it is exact for every pair within
+        // the list radius (Geom::oix); otherwise the minimum image per pair
         float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
         float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
         float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
@@ -742,9 +762,9 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
         float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
               sz = g.bz * rintf(dcz * g.ibz);
         bool one_image =
-            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
-            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
-            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < g.oix &&
+            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < g.oiy &&
+            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < g.oiz;
         float4 ps = pi;
         if (one_image) {
             ps.x -= sx;
@@ -888,7 +908,8 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
             if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
         }
         if (jb == ib) em |= (2ull << lane) - 1ull;
-        // one periodic shift for the whole tile when both boxes fit in half a box
+        // one periodic shift for the whole tile when it is exact for every pair within
+        // the list radius (Geom::oix); otherwise the minimum image per pair
         float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
         float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
         float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
@@ -896,9 +917,9 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
               sz = g.bz * rintf(dcz * g.ibz);
         bool one_image =
-            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
-            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
-            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+            fabsf(dcx - sx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < g.oix &&
+            fabsf(dcy - sy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < g.oiy &&
+            fabsf(dcz - sz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < g.oiz;
         float4 ps = pi;
         if (one_image) {
             ps.x -= sx;
@@ -1404,9 +1425,9 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_grp(
             float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
             float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
             float dcz = 0.5f * (bli.z + bhi.z - blj.z - bhj.z);
-            bool plain = fabsf(dcx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < 0.5f * g.bx - 1e-3f &&
-                         fabsf(dcy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < 0.5f * g.by - 1e-3f &&
-                         fabsf(dcz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < 0.5f * g.bz - 1e-3f;
+            bool plain = fabsf(dcx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < g.oix &&
+                         fabsf(dcy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < g.oiy &&
+                         fabsf(dcz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < g.oiz;
             cand = (plain ? 0 : 1) == pass && bbox_gap2(bli, bhi, blj, bhj, g) < list_r2;
         }
         return __ballot(cand);
@@ -1500,6 +1521,133 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_grp(
     double es = wave_sum(vi ? (acc.e + g.krf * aqr2 - g.crf * aq) : 0.0);
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(&flags[r], 2);
     if (lane == 0) e_pair[(size_t)r * NB + ib] = es;
+}
+
+// ---------------------------------------------------------------------------
+// Pair energies in FP64 (the energy evaluations: U0 and with it U1 = U0 + u and the
+// potential energy).  The force kernels compute in FP32 from FP32 coordinates, whose
+// rounding (an ulp is ~5e-7 nm at 4-8 nm) and arithmetic leave ~2^-24 of the summed pair
+// magnitudes in U0; where U0 is a small net of large terms that exceeds 1e-5 of U0.  This
+// kernel takes the FP64 positions and parameters of every pair with r < rc once, in
+// Newton-3 order over the half tile lists, so U0 is exact to FP64 rounding.
+//
+// One wave per (replica, i-block, part): part p of `split` takes tiles p, p + split, ..
+// Per tile the wave screens the mask's j atoms against its 64 i atoms in FP32 (the FP32
+// slot records, cutoff widened by 1e-4 nm so no pair inside rc is dropped) and queues
+// the surviving (i, j) lane pairs in LDS; every 64 queued pairs are evaluated one per
+// lane in FP64 (minimum image, exact cutoff test, LJ + reaction field), so the FP64 work
+// is spent on the ~25% of screened pairs that lie inside the cutoff.  half = 0: full tile
+// lists (the non-Newton kernel's), of which only tiles with j-block >= i-block are taken.
+// ---------------------------------------------------------------------------
+struct Rec64 {
+    double x, y, z, qk, sh, es;  // position; q sqrt(k_e), sigma/2, 2 sqrt(eps)
+};
+constexpr int E64_SPLIT = 4;  // waves (parts) per i-block of the FP64 energy kernel
+
+__global__ __launch_bounds__(64) void k_energy64(
+    int R, int N, int Np, int NB, int maxj, int split, int half, const float4* __restrict__ arec,
+    const double* __restrict__ x64, const double4* __restrict__ par64,
+    const int* __restrict__ orig_s, const int* __restrict__ slot_of,
+    const int* __restrict__ tiles, const unsigned long long* __restrict__ tmask,
+    const int* __restrict__ ntiles, const int* __restrict__ ex_start,
+    const int* __restrict__ ex_list, int max_ex, double* __restrict__ e_out, Geom g) {
+    __shared__ Rec64 irec[64], jrec[64];
+    __shared__ float4 jpos[64];
+    __shared__ unsigned short queue[128];
+    extern __shared__ int exs64[];  // [max_ex][64] exclusion partners' slots
+    const int w = xcd_linear_id(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    if (w >= R * NB * split) return;
+    const int part = w % split, rb = w / split;
+    const int r = rb / NB, ib = rb - r * NB;
+    const float4* A = arec + (size_t)r * Np * 2;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    const int* O = orig_s + (size_t)r * Np;
+    const int islot = ib * 64 + lane;
+    const bool vi = islot < N;
+    // this lane's i atom: FP32 record for the screen, FP64 record for the pairs
+    float4 pi = A[2 * islot];
+    {
+        int a = vi ? O[islot] : O[ib * 64];
+        double4 pr = par64[a];
+        irec[lane] = Rec64{X[a], X[Np + a], X[2 * Np + a], pr.x, pr.y, pr.z};
+    }
+    int nex = 0;
+    if (vi) {
+        int ai = O[islot];
+        int b0 = ex_start[ai];
+        nex = ex_start[ai + 1] - b0;
+        for (int k = 0; k < nex; k++) exs64[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+    }
+    const float rcs = sqrtf(g.rc2f) + 1e-4f;
+    const float rc2s = rcs * rcs;
+    const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
+    const unsigned long long* tmk = tmask + ((size_t)r * NB + ib) * maxj;
+    const int nt = ntiles[(size_t)r * NB + ib];
+    double e = 0.0;
+    unsigned head = 0, tail = 0;  // queue positions (wave-uniform)
+    // the queued pairs head .. head + n - 1 (n <= 64), one per lane
+    auto evaluate = [&](unsigned n) {
+        if ((unsigned)lane < n) {
+            unsigned q = queue[(head + lane) & 127u];
+            const Rec64 I = irec[q & 63u], J = jrec[q >> 6];
+            double dx = min_image(I.x - J.x, g.Lx, g.iLx);
+            double dy = min_image(I.y - J.y, g.Ly, g.iLy);
+            double dz = min_image(I.z - J.z, g.Lz, g.iLz);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < g.rc2) {
+                double ir = rsqrt(r2);
+                double ir2 = ir * ir;
+                double sig = I.sh + J.sh;
+                double s2 = sig * sig * ir2;
+                double s6 = s2 * s2 * s2;
+                double qq = I.qk * J.qk;
+                e += (I.es * J.es) * s6 * (s6 - 1.0) + qq * (ir + g.krf * r2 - g.crf);
+            }
+        }
+        head += n;
+    };
+    for (int t = part; t < nt; t += split) {
+        const int jb = tl[t];
+        if (!half && jb < ib) continue;  // full lists: each block pair once
+        unsigned long long am = tmk[t];
+        const int js = jb * 64 + lane;
+        __syncthreads();  // the previous tile's records are no longer read
+        if (js < N) {
+            int a = O[js];
+            double4 pr = par64[a];
+            jrec[lane] = Rec64{X[a], X[Np + a], X[2 * Np + a], pr.x, pr.y, pr.z};
+            jpos[lane] = A[2 * js];
+        }
+        unsigned long long em = 0;
+        for (int k = 0; k < nex; k++) {
+            int sl = exs64[k * 64 + lane];
+            if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+        }
+        if (jb == ib) em |= (2ull << lane) - 1ull;  // the diagonal tile: j > i only
+        __syncthreads();
+        while (am) {
+            const int j = next_bit(am);
+            const float4 pj = jpos[j];
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            dx -= g.bx * rintf(dx * g.ibx);
+            dy -= g.by * rintf(dy * g.iby);
+            dz -= g.bz * rintf(dz * g.ibz);
+            const bool in = vi && !((em >> j) & 1ull) && dx * dx + dy * dy + dz * dz < rc2s;
+            const unsigned long long m = __ballot(in);
+            if (in) queue[(tail + (unsigned)lane_rank(m)) & 127u] = (unsigned short)(lane | (j << 6));
+            tail += (unsigned)__popcll(m);
+            if (tail - head >= 64u) {
+                __syncthreads();  // the queue entries are written
+                evaluate(64u);
+                __syncthreads();  // and read before they are overwritten
+            }
+        }
+        __syncthreads();
+        evaluate(tail - head);  // the tile's remaining pairs (its j records change next)
+    }
+    e = wave_sum(e);
+    if (lane == 0) e_out[(size_t)rb * split + part] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -2171,8 +2319,13 @@ __device__ __forceinline__ bool excl_with(const int* __restrict__ ex_start,
 }
 
 // lanes = displaced atoms (ligand block lb: entries lb_start[lb] .. + lb_count[lb] of
-// lig, all of one displacement class), j = undisplaced atoms of candidate block c.
-__global__ __launch_bounds__(64) void k_deltaA(
+// lig, all of one displacement class), j = undisplaced atoms of candidate block c.  DW
+// waves per workgroup share the 64 staged j atoms (wave w takes j = w, w + DW, ..), so a
+// workgroup's serial chain of FP64 pair evaluations is DW times shorter; their partials
+// are added in wave order (deterministic).
+constexpr int DELTA_WAVES = 4;
+template <int DW>
+__global__ __launch_bounds__(64 * DW) void k_deltaA(
     int N, int Np, int L, int LB, const int* __restrict__ lig,
     const int* __restrict__ lb_start, const int* __restrict__ lb_count,
     const int* __restrict__ cls,
@@ -2182,17 +2335,20 @@ __global__ __launch_bounds__(64) void k_deltaA(
     const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
     const int* __restrict__ ncand, int maxc, double4* __restrict__ dA, Geom g,
     unsigned long long* __restrict__ pair_count) {
-    int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z, lane = threadIdx.x;
+    int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (c >= ncand[r]) return;  // uniform over the workgroup
     __shared__ AtomTile64 T;
+    __shared__ double4 part[DW][64];
     const double* X = x64 + (size_t)r * 3 * Np;
     int li = lb_start[lb] + lane;
     bool v = lane < lb_count[lb];
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     int jb = cand[(size_t)r * maxc + c];
     int jbeg = jb * 64, jn = min(64, N - jbeg);
-    if (lane < jn)
-        stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + jbeg + lane], cls, q64, sig64, eps64);
+    if ((int)threadIdx.x < jn)
+        stage_atom(T, threadIdx.x, X, Np, orig_s[(size_t)r * Np + jbeg + threadIdx.x], cls, q64,
+                   sig64, eps64);
     bool exI = v && excl_with(ex_start, ex_list, cls, I.idx, false);
     // boxes of this block of displaced atoms at x and at x+d (lanes past the block's
     // count repeat its first atom): j atoms beyond the cutoff of both boxes
@@ -2207,7 +2363,7 @@ __global__ __launch_bounds__(64) void k_deltaA(
     __syncthreads();
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cnt = 0;
-    for (int jj = 0; jj < jn; jj++) {
+    for (int jj = wv; jj < jn; jj += DW) {
         if (T.cls[jj] != 0) continue;
         Atom64 J = tile_atom(T, jj, cdispl);
         if (point_box_gap2(J.x, J.y, J.z, b0lx, b0ly, b0lz, b0hx, b0hy, b0hz, g) >= cut2 &&
@@ -2224,7 +2380,19 @@ __global__ __launch_bounds__(64) void k_deltaA(
         if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(s + 0.5));
         return;
     }
-    dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane] = make_double4(fx, fy, fz, du);
+    part[wv][lane] = make_double4(fx, fy, fz, du);
+    __syncthreads();
+    if (wv == 0) {
+        double4 t = part[0][lane];
+        for (int k = 1; k < DW; k++) {
+            double4 p = part[k][lane];
+            t.x += p.x;
+            t.y += p.y;
+            t.z += p.z;
+            t.w += p.w;
+        }
+        dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane] = t;
+    }
 }
 
 // per displaced atom: candidate partials in order, plus pairs with displaced atoms of a
@@ -2278,15 +2446,19 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     if (lane == 0) u_part[(size_t)r * LB + lb] = us;
 }
 
-// lanes = atoms of candidate block c (undisplaced only), loop over the displaced atoms.
-__global__ __launch_bounds__(64) void k_deltaB(
+// lanes = atoms of candidate block c (undisplaced only), loop over the displaced atoms
+// (DW waves per workgroup, wave w taking displaced atoms w, w + DW, .. of each staged
+// chunk; their partials are added in wave order).
+template <int DW>
+__global__ __launch_bounds__(64 * DW) void k_deltaB(
     int N, int Np, int L, const int* __restrict__ lig, const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ q64,
     const double* __restrict__ sig64, const double* __restrict__ eps64,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
     const int* __restrict__ ncand, int maxc, double4* __restrict__ dB, Geom g) {
-    int c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    int c = blockIdx.x, r = blockIdx.y;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (c >= ncand[r]) return;
     const double* X = x64 + (size_t)r * 3 * Np;
     int jb = cand[(size_t)r * maxc + c];
@@ -2309,13 +2481,15 @@ __global__ __launch_bounds__(64) void k_deltaB(
     const bool any_env = __ballot(env) != 0ull;
     const double cut2 = g.rc2 * (1.0 + 1e-9);
     __shared__ AtomTile64 T;
+    __shared__ double4 part[DW][64];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int base = 0; base < L && any_env; base += 64) {
         int n = min(64, L - base);
         __syncthreads();  // the previous chunk has been read
-        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, q64, sig64, eps64);
+        if ((int)threadIdx.x < n)
+            stage_atom(T, threadIdx.x, X, Np, lig[base + threadIdx.x], cls, q64, sig64, eps64);
         __syncthreads();
-        for (int k = 0; k < n; k++) {
+        for (int k = wv; k < n; k += DW) {
             Atom64 I = tile_atom(T, k, cdispl);
             if (point_box_gap2(I.x, I.y, I.z, blx, bly, blz, bhx, bhy, bhz, g) >= cut2 &&
                 point_box_gap2(I.x + I.dx, I.y + I.dy, I.z + I.dz, dlx, dly, dlz, dhx, dhy,
@@ -2326,7 +2500,18 @@ __global__ __launch_bounds__(64) void k_deltaB(
             delta_pair(J, I, g, fx, fy, fz, &iny);
         }
     }
-    dB[((size_t)r * maxc + c) * 64 + lane] = make_double4(fx, fy, fz, 0.0);
+    part[wv][lane] = make_double4(fx, fy, fz, 0.0);
+    __syncthreads();
+    if (wv == 0) {
+        double4 t = part[0][lane];
+        for (int k = 1; k < DW; k++) {
+            double4 p = part[k][lane];
+            t.x += p.x;
+            t.y += p.y;
+            t.z += p.z;
+        }
+        dB[((size_t)r * maxc + c) * 64 + lane] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -273,7 +273,12 @@ class DesmondDMSFile:
         is one of NONBONDED_METHODS (OpenMM's names); nonbondedCutoff in nm."""
         if nonbondedMethod not in NONBONDED_METHODS:
             raise ValueError("Illegal nonbonded method %r" % (nonbondedMethod,))
-        opls = self._check_unsupported() or OPLS
+        # The file's vdw_rule is checked (an unknown rule raises) but, as in the
+        # reference, only the OPLS argument selects geometric combining: its reader records
+        # a "geometric" rule (desmonddmsfile75.py:1061-1063) and then builds the forces
+        # with the createSystem argument (:409, :415), Lorentz-Berthelot by default.
+        self._check_unsupported()
+        opls = bool(OPLS)
         box = self.topology.box
         if box is None and nonbondedMethod in ("CutoffPeriodic", "Ewald", "PME", "LJPME"):
             raise ValueError("Illegal nonbonded method for a non-periodic system")
@@ -425,17 +430,26 @@ class DesmondDMSFile:
 
 
 def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=None,
-                  ligand=None, eps_rf: float = 78.5, cutoff: Optional[float] = None,
+                  ligand=None, eps_rf: Optional[float] = None, cutoff: Optional[float] = None,
                   name: str = ""):
     """An SDMSystem of the engine from a DMS force field: the nonbonded force with the
     reaction field (dielectric eps_rf) in place of Ewald/PME, which the engine does not
     have; bonds, angles and exclusions as SDMSystem arrays; torsions, impropers,
     positional restraints, 1-4 exceptions and constraints as `terms`.  displacement:
-    (n, 3) nm per atom (zeros by default); ligand: the displaced atom indices."""
+    (n, 3) nm per atom (zeros by default); ligand: the displaced atom indices.
+
+    CutoffNonPeriodic (the reference's vacuum example, example/test.py:63) is evaluated
+    as OpenMM does it: no periodic images, the reaction field with OpenMM's default
+    dielectric 78.3 (the reference's reader never sets another).  For the periodic
+    methods eps_rf defaults to 78.5, the dielectric of the benchmark workloads."""
     from .systems import SDMSystem  # local: systems does not depend on this module
-    if ff.box is None or ff.nonbonded_method in ("NoCutoff", "CutoffNonPeriodic"):
-        raise NotImplementedError("the engine evaluates periodic systems only "
-                                  "(CutoffPeriodic with the reaction field)")
+    periodic = ff.nonbonded_method != "CutoffNonPeriodic"
+    if ff.nonbonded_method == "NoCutoff" or (periodic and ff.box is None):
+        raise NotImplementedError("the engine evaluates cutoff systems only (CutoffPeriodic "
+                                  "with the reaction field in place of Ewald/PME, or "
+                                  "CutoffNonPeriodic)")
+    if eps_rf is None:
+        eps_rf = 78.5 if periodic else 78.3
     if ff.opls:
         raise NotImplementedError("geometric (OPLS) combining rules are not supported: the "
                                   "pair kernels combine sigma arithmetically")
@@ -448,10 +462,11 @@ def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=No
     if ligand is None:
         ligand = np.nonzero(np.any(disp != 0.0, axis=1))[0]
     return SDMSystem(
-        box=ff.box.copy(), cutoff=float(ff.cutoff if cutoff is None else cutoff),
+        box=ff.box.copy() if periodic else np.zeros(3),
+        cutoff=float(ff.cutoff if cutoff is None else cutoff),
         eps_rf=float(eps_rf), charge=ff.charge.copy(), sigma=ff.sigma.copy(),
         epsilon=ff.epsilon.copy(), mass=ff.mass.copy(), exclusions=ff.exclusions.copy(),
         bonds=ff.bonds.copy(), bond_r0=ff.bond_r0.copy(), bond_k=ff.bond_k.copy(),
         angles=ff.angles.copy(), angle_t0=ff.angle_t0.copy(), angle_k=ff.angle_k.copy(),
-        displacement=disp, positions=x, velocities=v, periodic=True,
+        displacement=disp, positions=x, velocities=v, periodic=periodic,
         ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff)

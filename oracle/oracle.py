@@ -90,6 +90,10 @@ def _load():
                                  C.POINTER(orc_params), _dp, _dp, _dp, C.POINTER(C.c_uint64),
                                  C.c_uint64, C.c_uint32, C.c_int, C.c_int,
                                  C.POINTER(orc_step_report)]
+    lib.orc_ref_step_replicas.argtypes = [C.POINTER(orc_nb), C.POINTER(orc_bonded_t), _dp, _dp,
+                                          C.POINTER(orc_params), _dp, _dp, _dp,
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int,
+                                          C.c_int, C.c_int, C.c_int]
     return lib
 
 
@@ -289,3 +293,25 @@ def ref_step(topo: Topology, params: orc_params, x, v, time, step, seed, replica
     if rc:
         raise ValueError("Unknown soft core method")
     return x.reshape(-1, 3), v.reshape(-1, 3), t.value, st.value, rep
+
+
+def ref_step_replicas(topo: Topology, params, xs, vs, seeds, nsteps, threads,
+                      use_cl_langevin=True):
+    """R replicas (lists of orc_params, (n,3) positions and velocities, seeds), each
+    advanced by nsteps reference steps from time 0, step 0, on `threads` OpenMP threads;
+    returns (xs, vs) as (R, n, 3) arrays."""
+    R = len(params)
+    pa = (orc_params * R)(*params)
+    x = np.ascontiguousarray(np.stack(xs), dtype=np.float64).copy()
+    v = np.ascontiguousarray(np.stack(vs), dtype=np.float64).copy()
+    t = np.zeros(R)
+    st = (C.c_uint64 * R)()
+    sd = (C.c_uint64 * R)(*[int(k) for k in seeds])
+    rc = lib().orc_ref_step_replicas(C.byref(topo.nb), C.byref(topo.bonded), topo.displ_p,
+                                     topo.mass_p, pa, x.ctypes.data_as(_dp),
+                                     v.ctypes.data_as(_dp), t.ctypes.data_as(_dp), st, sd, R,
+                                     nsteps, int(use_cl_langevin), int(threads))
+    if rc:
+        raise ValueError("Unknown soft core method")
+    return x, v
+

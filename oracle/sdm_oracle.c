@@ -666,3 +666,20 @@ int orc_ref_step(const orc_nb* nb, const orc_bonded_t* b, const double* displ,
     free(xi);
     return rc;
 }
+
+/* R independent replicas of orc_ref_step on `threads` OpenMP threads, one replica per
+   thread at a time (the CPU baseline's one-replica-per-core form).  x, v: [R][3n];
+   params, time, step, seed: [R].  Replica r draws the noise of replica index r. */
+int orc_ref_step_replicas(const orc_nb* nb, const orc_bonded_t* b, const double* displ,
+                          const double* mass, orc_params* params, double* x, double* v,
+                          double* time, uint64_t* step, const uint64_t* seed, int R,
+                          int nsteps, int use_cl_langevin, int threads) {
+    size_t n3 = 3 * (size_t)nb->n;
+    int rc = 0;
+#pragma omp parallel for num_threads(threads) schedule(static, 1) reduction(| : rc)
+    for (int r = 0; r < R; r++)
+        rc |= orc_ref_step(nb, b, displ, mass, &params[r], x + r * n3, v + r * n3, &time[r],
+                           &step[r], seed[r], (uint32_t)r, nsteps, use_cl_langevin, NULL);
+    return rc;
+}
+

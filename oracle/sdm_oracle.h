@@ -164,6 +164,11 @@ int orc_ref_step(const orc_nb* nb, const orc_bonded_t* b, const double* displ,
                  const double* mass, orc_params* p, double* x, double* v, double* time,
                  uint64_t* step, uint64_t seed, uint32_t replica, int nsteps,
                  int use_cl_langevin, orc_step_report* last);
+/* R replicas of orc_ref_step on `threads` OpenMP threads (one replica per thread). */
+int orc_ref_step_replicas(const orc_nb* nb, const orc_bonded_t* b, const double* displ,
+                          const double* mass, orc_params* params, double* x, double* v,
+                          double* time, uint64_t* step, const uint64_t* seed, int R,
+                          int nsteps, int use_cl_langevin, int threads);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,9 @@ def dihedral(p1, p2, p3, p4):
 
 
 def _min_image(d, box):
+    """Minimum image; no image for a non-periodic system (box None or zeros)."""
+    if box is None or not np.any(np.asarray(box)):
+        return d
     return d - box * np.round(d / box)
 
 

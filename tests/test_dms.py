@@ -144,8 +144,11 @@ def test_multi_file_offsets(tmp_path):
 
 
 def test_geometric_rule_and_rejections(tmp_path):
-    ff = DesmondDMSFile(_make_dms(str(tmp_path / "g.dms"), vdw_rule="geometric")).createSystem()
-    assert ff.opls
+    # a "geometric" file is read with Lorentz-Berthelot unless OPLS=True is asked for, as
+    # the reference's createSystem builds its forces from its OPLS argument alone
+    path = _make_dms(str(tmp_path / "g.dms"), vdw_rule="geometric")
+    assert not DesmondDMSFile(path).createSystem().opls
+    assert DesmondDMSFile(path).createSystem(OPLS=True).opls
     with pytest.raises(NotImplementedError):
         DesmondDMSFile(_make_dms(str(tmp_path / "u.dms"), vdw_rule="harmonic")).createSystem()
     with pytest.raises(NotImplementedError):
