@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
+    ap.add_argument("--side-fork", type=int, default=None,
+                    help="profiling modes: placement of the side work (sdm_tuning.side_fork)")
     ap.add_argument("--profile-large", type=int, default=0,
                     help="profiling mode: N eager steps of the 8-replica 99,920-atom ladder "
                          "(its benchmark tuning, kernels launched one by one), no other legs")
@@ -175,6 +177,8 @@ def main():
     if a.profile_large:
         big = large_ladder_context(systems, _native)
         big.set_tuning(**LARGE_TUNING, use_graph=0)
+        if a.side_fork is not None:
+            big.set_tuning(side_fork=a.side_fork)
         big.step(a.profile_large)
         e5 = big.energies()
         print(json.dumps({"profile_large_steps": a.profile_large, "step": e5[0]["step"],
@@ -203,7 +207,9 @@ def main():
             print(json.dumps({"pair_kernel_ms_events": ms, "tuning": ctx.get_tuning()}))
         else:
             # one launch per kernel (no graph) so a profiler attributes every dispatch
-            ctx.set_tuning(use_graph=0)
+            ctx.set_tuning(**LADDER_TUNING)
+            if a.side_fork is not None:
+                ctx.set_tuning(side_fork=a.side_fork)
             ctx.step(a.profile)
         e = ctx.energies()
         # calibration for the HBM-traffic counters: a fill that writes 64 MiB and a sum

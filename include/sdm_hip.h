@@ -161,9 +161,11 @@ typedef struct {
                                      and pair energies are summed on their own before they
                                      are added up. */
     int32_t side_fork;            /* where the bonded terms and the displaced-state chain
-                                     leave the main stream: 0 after the tile lists (beside
-                                     the pair kernel), 1 after the slot order (beside the
-                                     tile-list build as well).  Same results. */
+                                     run: 0 on two side streams forked after the tile lists
+                                     (beside the pair kernel), 1 forked after the slot order
+                                     (beside the tile-list build as well), 2 on the main
+                                     stream between the tile lists and the pair kernel.
+                                     Same results. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -367,8 +369,8 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 /* Launch-configuration search (project-written autotuner): times each candidate (pair
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
    for `steps` steps, re-times the three fastest for 3 x `steps` steps, each with and
-   without graph replay and with either fork point of the side work, and keeps the
-   fastest.  The state the context had before the
+   without graph replay and with each placement of the side work (side_fork), and
+   keeps the fastest.  The state the context had before the
    call is restored. */
 int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
 int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);

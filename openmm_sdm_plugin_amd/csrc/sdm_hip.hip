@@ -92,6 +92,7 @@ struct sdm_context {
     float *qs = nullptr, *sh = nullptr, *se = nullptr;
     double *q64 = nullptr, *sig64 = nullptr, *eps64 = nullptr, *invm = nullptr, *mass = nullptr;
     double4* par64 = nullptr;  // [Np] (q sqrt(k_e), sigma/2, 2 sqrt(eps), 0): FP64 energies
+    Rec64* rec64 = nullptr;    // [R][Np] slot-ordered FP64 records of the energy kernel
     int *ex_start = nullptr, *ex_list = nullptr, *cls = nullptr, *lig = nullptr, *cellrank = nullptr;
     // blocks of the displaced-atom list: entries lb_start[b] .. + lb_count[b] (<= 64),
     // all of one displacement class, so a block's bounding box covers one group of atoms
@@ -101,6 +102,10 @@ struct sdm_context {
     double *bond_r0 = nullptr, *bond_k = nullptr;
     int* ang_atoms = nullptr;
     double *ang_t0 = nullptr, *ang_k = nullptr;
+    // per-atom lists of the bonds and angles each atom belongs to (k_bonded's gather):
+    // entries (term << 1 | role) and (term << 2 | role) in term order
+    int *bstart = nullptr, *bcode = nullptr, *astart = nullptr, *acode = nullptr;
+    double* Fb = nullptr;  // [R][3][Np] FP64 bonded forces, atom order
     // per replica
     double *x64 = nullptr, *v64 = nullptr;
     float4* posq_s = nullptr;
@@ -138,6 +143,10 @@ struct sdm_context {
     int cm_every = 0;  // centre-of-mass motion removal interval (0: never)
     int *cand = nullptr, *ncand = nullptr;
     double4 *dA = nullptr, *dlig = nullptr, *dB = nullptr;
+    // [R][maxc][L] 64-bit lists: the undisplaced slots of candidate block c that displaced
+    // atom li pairs with (k_denv's screen, read by k_dlig)
+    unsigned long long* mlig = nullptr;
+    int nchunk = 1;  // k_dlig chunks of candidate blocks
     DevState* state = nullptr;
     DevRecord* rec = nullptr;
     int* flags = nullptr;
@@ -338,12 +347,15 @@ dim3 g2(int n, int per, int R) { return dim3((unsigned)((n + per - 1) / per), (u
 
 // slot order (a full spatial sort, or the last sort's order refreshed) and block boxes
 // for every replica
-void enqueue_order(sdm_context* c, bool full) {
+// energy: also the slot-ordered FP64 records of the energy kernel
+void enqueue_order(sdm_context* c, bool full, bool energy = false) {
     hipStream_t s = c->stream;
     int N = c->N, Np = c->Np, R = c->R;
+    Rec64* rec = energy ? c->rec64 : nullptr;
     if (!full) {
         k_refresh<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->x64, c->orig_s, c->qs, c->g,
-                                                c->posq_s, c->arec, c->F, c->bb_lo, c->bb_hi);
+                                                c->posq_s, c->arec, c->F, c->bb_lo, c->bb_hi,
+                                                c->par64, rec);
     } else {
     k_bin<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->x64, c->key, c->ccount, c->cellrank, c->g,
                                         c->ncx, c->ncy, c->ncz, c->NC);
@@ -351,7 +363,7 @@ void enqueue_order(sdm_context* c, bool full) {
     k_scatter<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->NC, c->key, c->cstart, c->cfill, c->orig_s);
     k_cellsort<<<g2(c->NC, 128, R), 128, 0, s>>>(N, Np, c->NC, c->cstart, c->orig_s, c->slot_of,
                                                 c->x64, c->qs, c->sh, c->se, c->posq_s,
-                                                c->par_s, c->arec, c->F, c->g);
+                                                c->par_s, c->arec, c->F, c->g, c->par64, rec);
     k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
     }
 }
@@ -385,15 +397,16 @@ void enqueue_sort(sdm_context* c, bool full) {
     enqueue_tiles(c, false);
 }
 
-// FP64 pair energies into e64 (needs the tile lists of this evaluation)
+// FP64 pair energies into e64 (needs the tile lists and FP64 records of this evaluation)
 void enqueue_energy64(sdm_context* c) {
     const int half = c->tune.pair_group != 0 ? 1 : c->tune.pair_newton3;
     unsigned waves = (unsigned)(c->R * c->NB * E64_SPLIT);
     size_t lds = (size_t)std::max(1, c->max_ex) * 64 * sizeof(int);
     k_energy64<<<waves, 64, lds, c->stream>>>(c->R, c->N, c->Np, c->NB, c->maxj, E64_SPLIT,
-                                              half, c->arec, c->x64, c->par64, c->orig_s,
-                                              c->slot_of, c->tiles, c->tmask, c->ntiles,
-                                              c->ex_start, c->ex_list, c->max_ex, c->e64, c->g);
+                                              half, c->arec, c->rec64, c->orig_s, c->slot_of,
+                                              c->tiles, c->tmask, c->ntiles, c->bb_lo,
+                                              c->bb_hi, c->ex_start, c->ex_list, c->max_ex,
+                                              c->e64, c->g);
 }
 
 // ring capacity (entries per group) of the grouped pair kernel
@@ -493,22 +506,29 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
 // pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
                   hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
-    hipStream_t s = c->stream, s1 = c->side[0], s2 = c->side[1];
+    // tuning.side_fork: the side work (bonded terms, restraints, force-field terms, the
+    // displaced-state chain) needs the slot order, block boxes and cleared accumulator,
+    // not the tile lists.  0: it forks onto the two side streams after the tile lists,
+    // 1: before them, 2: it runs on the main stream between the tile lists and the pair
+    // kernel (no fork; while the pair kernel fills the GPU, side kernels beside it wait
+    // for free wave slots and can outlast it)
+    const int mode = c->tune.side_fork;
+    hipStream_t s = c->stream;
+    hipStream_t s1 = mode == 2 ? s : c->side[0], s2 = mode == 2 ? s : c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
-    enqueue_order(c, full);
-    // the side work needs the slot order, block boxes and cleared accumulator, not the
-    // tile lists: it forks before or after their build (tuning.side_fork)
-    if (!c->tune.side_fork) enqueue_tiles(c, energy);
-    (void)hipEventRecord(c->ev_fork, s);
-    (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
-    (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
-    if (c->tune.side_fork) enqueue_tiles(c, energy);
+    enqueue_order(c, full, energy);
+    if (mode != 1) enqueue_tiles(c, energy);
+    if (mode != 2) {
+        (void)hipEventRecord(c->ev_fork, s);
+        (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
+        (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
+    }
+    if (mode == 1) enqueue_tiles(c, energy);
     if (c->nbond + c->nang > 0)
-        k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(Np, c->nbond, c->nang, c->bond_atoms, c->bond_r0,
-                                                  c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k,
-                                                  c->x64, components ? nullptr : c->slot_of,
-                                                  components ? c->Fb_sep : c->F, c->e_bond,
-                                                  c->nbw, c->flags);
+        k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(N, Np, c->bstart, c->bcode, c->astart, c->acode,
+                                                  c->bond_atoms, c->bond_r0, c->bond_k,
+                                                  c->ang_atoms, c->ang_t0, c->ang_k, c->x64,
+                                                  c->Fb, c->e_bond, c->nbw, c->flags);
     if (c->ncen + c->nali > 0)
         k_restraints<<<dim3(c->ncen + c->nali, R), 64, 0, s1>>>(
             Np, c->ncen, c->ncen + c->nali, c->rcen, c->rali, c->ratom, c->rw, c->x64,
@@ -520,32 +540,33 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             components ? c->Fb_sep : c->F, c->slot_of, c->F, c->e_t1, c->e_t2, c->ntb, c->g,
             c->flags);
     if (c->L > 0) {
-        k_ligcand<<<R, 64, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
+        k_ligcand<<<R, LIGCAND_THREADS, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
                                     c->g, c->flags);
-        k_deltaA<DELTA_WAVES><<<dim3(c->maxc, c->LB, R), 64 * DELTA_WAVES, 0, s2>>>(
+        k_denv<<<dim3(c->maxc, R), 64, 0, s2>>>(
             N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64,
-            c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
-            nullptr);
+            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand,
+            c->maxc, c->mlig, c->dB, c->g);
+        k_dlig<<<dim3(c->nchunk, c->LB, R), 64, 0, s2>>>(
+            N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
+            c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig,
+            c->nchunk, c->dA, c->g, nullptr);
         k_deltaA_red<<<dim3(c->LB, R), 64, 0, s2>>>(
             Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64, c->ex_start,
-            c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g, nullptr);
-        k_deltaB<DELTA_WAVES><<<dim3(c->maxc, R), 64 * DELTA_WAVES, 0, s2>>>(N, Np, c->L, c->lig, c->cls, c->cdispl, c->q64,
-                                                  c->sig64, c->eps64, c->ex_start, c->ex_list,
-                                                  c->x64, c->orig_s, c->cand, c->ncand, c->maxc,
-                                                  c->dB, c->g);
+            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->nchunk, c->dA,
+            c->dlig, c->u_part, c->g, nullptr);
     }
     if (pe0) (void)hipEventRecord(pe0, s);
     // forces in FP32; the pair energies, when asked for, in FP64 by their own kernel
     enqueue_pair(c, false, false);
     if (pe1) (void)hipEventRecord(pe1, s);
     if (energy) enqueue_energy64(c);
-    (void)hipEventRecord(c->ev_join[0], s1);
-    (void)hipEventRecord(c->ev_join[1], s2);
-    (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
-    (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
+    if (mode != 2) {
+        (void)hipEventRecord(c->ev_join[0], s1);
+        (void)hipEventRecord(c->ev_join[1], s2);
+        (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
+        (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
+    }
     k_scalar<<<R, 256, 0, s>>>(c->NB * E64_SPLIT, c->nbw, c->LB, c->e64, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
@@ -562,9 +583,9 @@ void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullp
     if (c->cm_every > 0)
         k_cm_remove<<<c->R, 1024, 0, c->stream>>>(c->N, c->Np, c->cm_every, c->mass, c->v64,
                                                    c->state);
-    k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->x64,
-                                                           c->v64, c->rec, c->state, c->slot_of,
-                                                           c->cflag, c->dxs);
+    k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->Fb,
+                                                           c->x64, c->v64, c->rec, c->state,
+                                                           c->slot_of, c->cflag, c->dxs);
     if (c->ncl > 0)
         k_constrain<<<g2(c->ncl, 256, c->R), 256, 0, c->stream>>>(
             c->Np, c->ncl, c->cl_start, c->cl_atoms, c->con_start, c->con_ij, c->con_d2,
@@ -723,6 +744,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         elist[ecount[a] + efill[a]++] = b;
         elist[ecount[b] + efill[b]++] = a;
     }
+    if (sys->n_bonds >= (1 << 28) || sys->n_angles >= (1 << 28))
+        return fail(SDM_ERR_INVALID, "at most 2^28 - 1 bonds and 2^28 - 1 angles");
     for (int k = 0; k < sys->n_bonds; k++)
         for (int j = 0; j < 2; j++)
             if (sys->bond_atoms[2 * k + j] < 0 || sys->bond_atoms[2 * k + j] >= N)
@@ -797,7 +820,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     c->ncls = (int)cd.size() / 3 - 1;
     c->nbond = sys->n_bonds;
     c->nang = sys->n_angles;
-    c->nbw = std::max(1, (c->nbond + c->nang + 255) / 256);
+    c->nbw = std::max(1, (N + 255) / 256);
     c->maxj = tile_capacity(sys, c->NB);
     c->maxc = std::min(c->NB, 256 * std::max(1, c->ncls) + 64);
     Geom& g = c->g;
@@ -851,15 +874,20 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(bond_atoms, 2 * std::max(1, c->nbond)); A(bond_r0, std::max(1, c->nbond));
     A(bond_k, std::max(1, c->nbond)); A(ang_atoms, 3 * std::max(1, c->nang));
     A(ang_t0, std::max(1, c->nang)); A(ang_k, std::max(1, c->nang));
+    A(bstart, N + 1); A(bcode, 2 * std::max(1, c->nbond));
+    A(astart, N + 1); A(acode, 3 * std::max(1, c->nang));
+    if (c->nbond + c->nang > 0) A(Fb, 3 * RNp);
     A(x64, 3 * RNp); A(v64, 3 * RNp); A(posq_s, RNp); A(par_s, RNp); A(arec, 2 * RNp);
     A(orig_s, RNp);
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
     A(e_pair, (size_t)R * NB * 8); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
-    A(e64, (size_t)R * NB * E64_SPLIT);
+    A(e64, (size_t)R * NB * E64_SPLIT); A(rec64, RNp);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
-    A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
+    c->nchunk = (c->maxc + DLIG_CB - 1) / DLIG_CB;
+    A(dA, (size_t)R * c->nchunk * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
+    A(mlig, (size_t)R * c->maxc * std::max(1, c->L));
     A(dB, (size_t)R * c->maxc * 64); A(state, R); A(rec, R); A(flags, R);
     A(uhist, (size_t)R * c->hist_cap); A(counters, 2 * (size_t)R); A(ke_out, R);
 #undef A
@@ -894,6 +922,27 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
             ((rc = upload(c, c->ang_atoms, sys->angle_atoms, 3 * (size_t)c->nang)) ||
              (rc = upload(c, c->ang_t0, sys->angle_theta0, c->nang)) ||
              (rc = upload(c, c->ang_k, sys->angle_k, c->nang))))
+            return bail(rc);
+        // per-atom term lists: counts, exclusive scan, then the entries in term order
+        std::vector<int> bst(N + 1, 0), ast(N + 1, 0);
+        for (int k = 0; k < c->nbond; k++)
+            for (int j = 0; j < 2; j++) bst[sys->bond_atoms[2 * k + j] + 1]++;
+        for (int k = 0; k < c->nang; k++)
+            for (int j = 0; j < 3; j++) ast[sys->angle_atoms[3 * k + j] + 1]++;
+        for (int i = 0; i < N; i++) {
+            bst[i + 1] += bst[i];
+            ast[i + 1] += ast[i];
+        }
+        std::vector<int> bco(std::max(1, 2 * c->nbond)), aco(std::max(1, 3 * c->nang));
+        std::vector<int> bfill(bst.begin(), bst.end() - 1), afill(ast.begin(), ast.end() - 1);
+        for (int k = 0; k < c->nbond; k++)
+            for (int j = 0; j < 2; j++) bco[bfill[sys->bond_atoms[2 * k + j]]++] = (k << 1) | j;
+        for (int k = 0; k < c->nang; k++)
+            for (int j = 0; j < 3; j++) aco[afill[sys->angle_atoms[3 * k + j]]++] = (k << 2) | j;
+        if ((rc = upload(c, c->bstart, bst.data(), N + 1)) ||
+            (rc = upload(c, c->bcode, bco.data(), bco.size())) ||
+            (rc = upload(c, c->astart, ast.data(), N + 1)) ||
+            (rc = upload(c, c->acode, aco.data(), aco.size())))
             return bail(rc);
         std::vector<DevState> st(R);
         sdm_params p = default_params();
@@ -1442,12 +1491,15 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
     size_t off = (size_t)r * 3 * Np;
     std::vector<long long> f(3 * (size_t)Np);
     std::vector<int> slot(Np);
+    std::vector<double> fbond(3 * (size_t)Np, 0.0);  // FP64 bonds and angles, atom order
     CHK(download(c, f.data(), c->F + off, f.size()));
     CHK(download(c, slot.data(), c->slot_of + (size_t)r * Np, Np));
+    if (c->Fb) CHK(download(c, fbond.data(), c->Fb + off, fbond.size()));
     if (which == SDM_FORCE_HYBRID && !c->components_valid) {
         for (int i = 0; i < N; i++)
             for (int d = 0; d < 3; d++)
-                out[3 * i + d] = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE;
+                out[3 * i + d] = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE +
+                                 fbond[(size_t)d * Np + i];
         return SDM_OK;
     }
     if (!c->components_valid)
@@ -1462,7 +1514,7 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
         for (int d = 0; d < 3; d++) {
             size_t k = (size_t)d * Np + i;
             double f0 = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE;
-            double b = (double)fb[k] / SDM_FIX_SCALE;
+            double b = (double)fb[k] / SDM_FIX_SCALE + fbond[k];
             double v = 0.0;
             switch (which) {
                 case SDM_FORCE_HYBRID: v = f0 + rec.sp * dF[k] + b; break;
@@ -1541,19 +1593,21 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
     enqueue_sort(c, take_order(c, false));
     enqueue_pair(c, true);
     if (c->L > 0) {
-        k_ligcand<<<c->R, 64, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
+        k_ligcand<<<c->R, LIGCAND_THREADS, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
                                       c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
                                       c->list_r2, c->g, c->flags);
-        k_deltaA<DELTA_WAVES><<<dim3(c->maxc, c->LB, c->R), 64 * DELTA_WAVES, 0, s>>>(
+        k_denv<<<dim3(c->maxc, c->R), 64, 0, s>>>(
             c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
-            c->q64, c->sig64, c->eps64,
-            c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->dA, c->g,
-            c->counters + c->R);
+            c->q64, c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand,
+            c->ncand, c->maxc, c->mlig, c->dB, c->g);
+        k_dlig<<<dim3(c->nchunk, c->LB, c->R), 64, 0, s>>>(
+            c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
+            c->q64, c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig,
+            c->nchunk, c->dA, c->g, c->counters + c->R);
         k_deltaA_red<<<dim3(c->LB, c->R), 64, 0, s>>>(
             c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64,
-            c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA, c->dlig, c->u_part, c->g,
-            c->counters + c->R);
+            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->nchunk, c->dA,
+            c->dlig, c->u_part, c->g, c->counters + c->R);
     }
     CHK(check_launch());
     std::vector<unsigned long long> h(n);
@@ -1625,8 +1679,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4 or 8");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
-    if (t->side_fork != 0 && t->side_fork != 1)
-        return fail(SDM_ERR_INVALID, "side_fork must be 0 or 1");
+    if (t->side_fork < 0 || t->side_fork > 2)
+        return fail(SDM_ERR_INVALID, "side_fork must be 0, 1 or 2");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -1709,7 +1763,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
     std::stable_sort(timed.begin(), timed.end(),
                      [](const auto& p, const auto& q) { return p.first < q.first; });
     for (size_t k = 0; k < timed.size() && k < 3 && rc == SDM_OK; k++)
-        for (int variant = 0; variant < 4 && rc == SDM_OK; variant++) {
+        for (int variant = 0; variant < 6 && rc == SDM_OK; variant++) {
             sdm_tuning t = timed[k].second;
             t.use_graph = variant & 1;
             t.side_fork = variant >> 1;

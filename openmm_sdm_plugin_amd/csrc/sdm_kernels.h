@@ -4,14 +4,16 @@
 // openmmapi/src/LangevinIntegratorSDM.cpp:153-183) is the launch sequence
 //
 //   k_bin -> k_scan -> k_scatter -> k_cellsort -> k_bbox -> k_tiles     spatial sort + tiles
-//   k_pair                 U0, F0: full LJ+RF sweep at x (FP32 math, fixed-point forces)
-//   k_bonded               Ub, Fb: harmonic bonds/angles at x (FP64, fixed-point atomics)
+//   k_pair                 F0: full LJ+RF sweep at x (FP32 math, fixed-point forces)
+//   k_energy64             U0 on the steps that report energies (FP64 pair sums)
+//   k_bonded               Ub, Fb: harmonic bonds/angles at x (FP64, gathered per atom)
 //   k_ligcand              j-blocks near the displaced atoms at x or at x+d
-//   k_deltaA, k_deltaA_red u and F1-F0 on displaced atoms (FP64, pairs with d_i != d_j)
-//   k_deltaB               F1-F0 on the surrounding atoms (FP64)
+//   k_denv                 screened pair lists; F1-F0 on the surrounding atoms (FP64)
+//   k_dlig, k_deltaA_red   u and F1-F0 on displaced atoms (FP64, pairs with d_i != d_j)
 //   k_scalar               SoftCoreF, bias W/W', PotEnergy, noneq schedule/work, sp
 //   k_mix                  F += sp (F1 - F0)
 //   k_langevin             leapfrog Langevin update, Philox noise (FP64)
+//   k_constrain            SHAKE on constrained clusters (FP64)
 //
 // U1 and F1 are never computed as a second full sweep: pairs whose displacements are
 // equal (d_i == d_j) have identical minimum-image distances at x and at x+d, so
@@ -43,6 +45,12 @@ struct Geom {
     // (wrapped, if periodic) coordinate.  Periodic: go = 0, gs = cells / L; otherwise
     // the grid spans a region around the positions (atoms outside take edge cells)
     double go[3], gs[3];
+};
+
+// Slot-ordered FP64 record of the energy kernel: the position in the same periodic image
+// as the slot's FP32 record (so one tile shift serves both), and the parameters.
+struct Rec64 {
+    double x, y, z, qk, sh, es;  // position; q sqrt(k_e), sigma/2, 2 sqrt(eps)
 };
 
 __device__ __forceinline__ int grid_cell(double w, double go, double gs, int n) {
@@ -122,7 +130,8 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
                            const double* __restrict__ x64, const float* __restrict__ qs,
                            const float* __restrict__ sh, const float* __restrict__ se,
                            float4* __restrict__ posq_s, float2* __restrict__ par_s,
-                           float4* __restrict__ arec, long long* __restrict__ F, Geom g) {
+                           float4* __restrict__ arec, long long* __restrict__ F, Geom g,
+                           const double4* __restrict__ par64, Rec64* __restrict__ rec) {
     int r = blockIdx.y;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= NC) return;
@@ -148,6 +157,10 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
         wz -= g.Lz * floor(wz * g.iLz);
         float4 pq = make_float4((float)wx, (float)wy, (float)wz, qs[i]);
         posq_s[(size_t)r * Np + s] = pq;
+        if (rec) {  // energy evaluations: the FP64 record in the same image
+            double4 pr = par64[i];
+            rec[(size_t)r * Np + s] = Rec64{wx, wy, wz, pr.x, pr.y, pr.z};
+        }
         par_s[(size_t)r * Np + s] = make_float2(sh[i], se[i]);
         // interleaved copy for the grouped pair kernel: one 32-byte record per slot
         arec[2 * ((size_t)r * Np + s)] = pq;
@@ -191,7 +204,9 @@ __global__ __launch_bounds__(64) void k_refresh(int N, int Np, int NB,
                                                 float4* __restrict__ arec,
                                                 long long* __restrict__ F,
                                                 float4* __restrict__ bb_lo,
-                                                float4* __restrict__ bb_hi) {
+                                                float4* __restrict__ bb_hi,
+                                                const double4* __restrict__ par64,
+                                                Rec64* __restrict__ rec) {
     int r = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
     int sl = b * 64 + lane;
     bool v = sl < N;  // slot b * 64 (lane 0) is always below N
@@ -211,6 +226,10 @@ __global__ __launch_bounds__(64) void k_refresh(int N, int Np, int NB,
     if (v) {
         posq_s[o] = p;
         arec[2 * o] = p;
+        if (rec) {  // energy evaluations: the FP64 record in the same image
+            double4 pr = par64[i];
+            rec[o] = Rec64{wx, wy, wz, pr.x, pr.y, pr.z};
+        }
         long long* Fr = F + (size_t)r * 3 * Np;
         Fr[sl] = 0;
         Fr[Np + sl] = 0;
@@ -1539,18 +1558,16 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_grp(
 // is spent on the ~25% of screened pairs that lie inside the cutoff.  half = 0: full tile
 // lists (the non-Newton kernel's), of which only tiles with j-block >= i-block are taken.
 // ---------------------------------------------------------------------------
-struct Rec64 {
-    double x, y, z, qk, sh, es;  // position; q sqrt(k_e), sigma/2, 2 sqrt(eps)
-};
 constexpr int E64_SPLIT = 4;  // waves (parts) per i-block of the FP64 energy kernel
 
 __global__ __launch_bounds__(64) void k_energy64(
     int R, int N, int Np, int NB, int maxj, int split, int half, const float4* __restrict__ arec,
-    const double* __restrict__ x64, const double4* __restrict__ par64,
-    const int* __restrict__ orig_s, const int* __restrict__ slot_of,
-    const int* __restrict__ tiles, const unsigned long long* __restrict__ tmask,
-    const int* __restrict__ ntiles, const int* __restrict__ ex_start,
-    const int* __restrict__ ex_list, int max_ex, double* __restrict__ e_out, Geom g) {
+    const Rec64* __restrict__ rec, const int* __restrict__ orig_s,
+    const int* __restrict__ slot_of, const int* __restrict__ tiles,
+    const unsigned long long* __restrict__ tmask, const int* __restrict__ ntiles,
+    const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int max_ex,
+    double* __restrict__ e_out, Geom g) {
     __shared__ Rec64 irec[64], jrec[64];
     __shared__ float4 jpos[64];
     __shared__ unsigned short queue[128];
@@ -1561,39 +1578,43 @@ __global__ __launch_bounds__(64) void k_energy64(
     const int part = w % split, rb = w / split;
     const int r = rb / NB, ib = rb - r * NB;
     const float4* A = arec + (size_t)r * Np * 2;
-    const double* X = x64 + (size_t)r * 3 * Np;
-    const int* O = orig_s + (size_t)r * Np;
+    const Rec64* RC = rec + (size_t)r * Np;
     const int islot = ib * 64 + lane;
     const bool vi = islot < N;
+    const bool full_block = ib * 64 + 63 < N;  // uniform: every lane holds an atom
     // this lane's i atom: FP32 record for the screen, FP64 record for the pairs
-    float4 pi = A[2 * islot];
-    {
-        int a = vi ? O[islot] : O[ib * 64];
-        double4 pr = par64[a];
-        irec[lane] = Rec64{X[a], X[Np + a], X[2 * Np + a], pr.x, pr.y, pr.z};
-    }
+    const float4 pi = A[2 * islot];
+    irec[lane] = RC[vi ? islot : ib * 64];
     int nex = 0;
     if (vi) {
-        int ai = O[islot];
+        int ai = orig_s[(size_t)r * Np + islot];
         int b0 = ex_start[ai];
         nex = ex_start[ai + 1] - b0;
         for (int k = 0; k < nex; k++) exs64[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
     }
     const float rcs = sqrtf(g.rc2f) + 1e-4f;
     const float rc2s = rcs * rcs;
+    const float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
     const int* tl = tiles + ((size_t)r * NB + ib) * maxj;
     const unsigned long long* tmk = tmask + ((size_t)r * NB + ib) * maxj;
     const int nt = ntiles[(size_t)r * NB + ib];
     double e = 0.0;
     unsigned head = 0, tail = 0;  // queue positions (wave-uniform)
-    // the queued pairs head .. head + n - 1 (n <= 64), one per lane
+    // the queued pairs head .. head + n - 1 (n <= 64), one per lane.  one_image: every
+    // pair of the tile takes the shift (sx, sy, sz) (records are in the FP32 records'
+    // images); otherwise the minimum image per pair
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    bool one_image = true;
     auto evaluate = [&](unsigned n) {
         if ((unsigned)lane < n) {
             unsigned q = queue[(head + lane) & 127u];
             const Rec64 I = irec[q & 63u], J = jrec[q >> 6];
-            double dx = min_image(I.x - J.x, g.Lx, g.iLx);
-            double dy = min_image(I.y - J.y, g.Ly, g.iLy);
-            double dz = min_image(I.z - J.z, g.Lz, g.iLz);
+            double dx = (I.x - sx) - J.x, dy = (I.y - sy) - J.y, dz = (I.z - sz) - J.z;
+            if (!one_image) {
+                dx = min_image(dx, g.Lx, g.iLx);
+                dy = min_image(dy, g.Ly, g.iLy);
+                dz = min_image(dz, g.Lz, g.iLz);
+            }
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < g.rc2) {
                 double ir = rsqrt(r2);
@@ -1607,6 +1628,29 @@ __global__ __launch_bounds__(64) void k_energy64(
         }
         head += n;
     };
+    // screen of one j atom against the wave's i atoms, queueing the pairs inside the
+    // widened cutoff.  MI: per-pair minimum image (otherwise the tile's shift is in ps);
+    // CHK: some lane is padding or has an exclusion partner in the tile
+    auto screen = [&](int j, float4 ps, unsigned long long em, auto mic, auto chk) {
+        constexpr bool MI = decltype(mic)::value, CHK = decltype(chk)::value;
+        const float4 pj = jpos[j];
+        float dx = ps.x - pj.x, dy = ps.y - pj.y, dz = ps.z - pj.z;
+        if (MI) {
+            dx -= g.bx * rintf(dx * g.ibx);
+            dy -= g.by * rintf(dy * g.iby);
+            dz -= g.bz * rintf(dz * g.ibz);
+        }
+        bool in = dx * dx + dy * dy + dz * dz < rc2s;
+        if (CHK) in = in && vi && !((em >> j) & 1ull);
+        const unsigned long long m = __ballot(in);
+        if (in) queue[(tail + (unsigned)lane_rank(m)) & 127u] = (unsigned short)(lane | (j << 6));
+        tail += (unsigned)__popcll(m);
+        if (tail - head >= 64u) {
+            __syncthreads();  // the queue entries are written
+            evaluate(64u);
+            __syncthreads();  // and read before they are overwritten
+        }
+    };
     for (int t = part; t < nt; t += split) {
         const int jb = tl[t];
         if (!half && jb < ib) continue;  // full lists: each block pair once
@@ -1614,9 +1658,7 @@ __global__ __launch_bounds__(64) void k_energy64(
         const int js = jb * 64 + lane;
         __syncthreads();  // the previous tile's records are no longer read
         if (js < N) {
-            int a = O[js];
-            double4 pr = par64[a];
-            jrec[lane] = Rec64{X[a], X[Np + a], X[2 * Np + a], pr.x, pr.y, pr.z};
+            jrec[lane] = RC[js];
             jpos[lane] = A[2 * js];
         }
         unsigned long long em = 0;
@@ -1625,23 +1667,38 @@ __global__ __launch_bounds__(64) void k_energy64(
             if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
         }
         if (jb == ib) em |= (2ull << lane) - 1ull;  // the diagonal tile: j > i only
+        const bool chk = !full_block || __ballot(em != 0ull) != 0ull;
+        // one periodic shift for the whole tile when it is exact for every pair within
+        // the list radius (Geom::oix), as in the pair kernel
+        const float4 blj = bb_lo[(size_t)r * NB + jb], bhj = bb_hi[(size_t)r * NB + jb];
+        const float dcx = 0.5f * (bli.x + bhi.x - blj.x - bhj.x);
+        const float dcy = 0.5f * (bli.y + bhi.y - blj.y - bhj.y);
+        const float dcz = 0.5f * (bli.z + bhi.z - blj.z - bhj.z);
+        const float kx = rintf(dcx * g.ibx), ky = rintf(dcy * g.iby), kz = rintf(dcz * g.ibz);
+        const float fsx = g.bx * kx, fsy = g.by * ky, fsz = g.bz * kz;
+        one_image = fabsf(dcx - fsx) + 0.5f * (bhi.x - bli.x + bhj.x - blj.x) < g.oix &&
+                    fabsf(dcy - fsy) + 0.5f * (bhi.y - bli.y + bhj.y - blj.y) < g.oiy &&
+                    fabsf(dcz - fsz) + 0.5f * (bhi.z - bli.z + bhj.z - blj.z) < g.oiz;
+        float4 ps = pi;
+        if (one_image) {
+            ps.x -= fsx;
+            ps.y -= fsy;
+            ps.z -= fsz;
+            sx = g.Lx * (double)kx;  // the same shift in FP64 (0 if not periodic)
+            sy = g.Ly * (double)ky;
+            sz = g.Lz * (double)kz;
+        } else {
+            sx = sy = sz = 0.0;
+        }
         __syncthreads();
-        while (am) {
-            const int j = next_bit(am);
-            const float4 pj = jpos[j];
-            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            dx -= g.bx * rintf(dx * g.ibx);
-            dy -= g.by * rintf(dy * g.iby);
-            dz -= g.bz * rintf(dz * g.ibz);
-            const bool in = vi && !((em >> j) & 1ull) && dx * dx + dy * dy + dz * dz < rc2s;
-            const unsigned long long m = __ballot(in);
-            if (in) queue[(tail + (unsigned)lane_rank(m)) & 127u] = (unsigned short)(lane | (j << 6));
-            tail += (unsigned)__popcll(m);
-            if (tail - head >= 64u) {
-                __syncthreads();  // the queue entries are written
-                evaluate(64u);
-                __syncthreads();  // and read before they are overwritten
-            }
+        using T = std::true_type;
+        using Fa = std::false_type;
+        if (one_image) {
+            if (chk) while (am) screen(next_bit(am), ps, em, Fa(), T());
+            else while (am) screen(next_bit(am), ps, em, Fa(), Fa());
+        } else {
+            if (chk) while (am) screen(next_bit(am), ps, em, T(), T());
+            else while (am) screen(next_bit(am), ps, em, T(), Fa());
         }
         __syncthreads();
         evaluate(tail - head);  // the tile's remaining pairs (its j records change next)
@@ -1651,67 +1708,73 @@ __global__ __launch_bounds__(64) void k_energy64(
 }
 
 // ---------------------------------------------------------------------------
-// group 1: harmonic bonds and angles at x (FP64), fixed-point atomics.
+// group 1: harmonic bonds and angles at x (FP64), gathered per atom.  One thread per
+// (atom, replica) evaluates every term its atom belongs to (a bond by both of its atoms,
+// an angle by all three) and keeps only its own atom's share, so the forces need no
+// atomics and are summed in a fixed order; a term's energy is counted by its first
+// atom.  bcode holds (bond << 1) | role and acode (angle << 2) | role, role being the
+// atom's position in the term; the lists of atom i are [bstart[i], bstart[i+1]) and
+// [astart[i], astart[i+1]).  The forces go to Fb (FP64, atom order, [R][3][Np]), which
+// the integrator adds to the fixed-point accumulator.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bonded(
-    int Np, int nbond, int nang, const int* __restrict__ bond_atoms,
-    const double* __restrict__ bond_r0, const double* __restrict__ bond_k,
-    const int* __restrict__ ang_atoms, const double* __restrict__ ang_t0,
-    const double* __restrict__ ang_k, const double* __restrict__ x64,
-    const int* __restrict__ slot_of, long long* __restrict__ F, double* __restrict__ e_bond,
+    int N, int Np, const int* __restrict__ bstart, const int* __restrict__ bcode,
+    const int* __restrict__ astart, const int* __restrict__ acode,
+    const int* __restrict__ bond_atoms, const double* __restrict__ bond_r0,
+    const double* __restrict__ bond_k, const int* __restrict__ ang_atoms,
+    const double* __restrict__ ang_t0, const double* __restrict__ ang_k,
+    const double* __restrict__ x64, double* __restrict__ Fb, double* __restrict__ e_bond,
     int nbw, int* __restrict__ flags) {
     __shared__ double red[256];
     int r = blockIdx.y;
-    int t = blockIdx.x * 256 + threadIdx.x;
+    int i = blockIdx.x * 256 + threadIdx.x;
     const double* X = x64 + (size_t)r * 3 * Np;
-    unsigned long long* Fr = (unsigned long long*)(F + (size_t)r * 3 * Np);
-    const int* S = slot_of ? slot_of + (size_t)r * Np : nullptr;
     double e = 0.0;
-    int bad = 0;
-    if (t < nbond) {
-        int i = bond_atoms[2 * t], j = bond_atoms[2 * t + 1];
-        int si = S ? S[i] : i, sj = S ? S[j] : j;
-        double dx = X[j] - X[i], dy = X[Np + j] - X[Np + i], dz = X[2 * Np + j] - X[2 * Np + i];
-        double rr = sqrt(dx * dx + dy * dy + dz * dz);
-        double dr = rr - bond_r0[t];
-        e = 0.5 * bond_k[t] * dr * dr;
-        double gg = rr > 0.0 ? bond_k[t] * dr / rr : 0.0;
-        long long fx = to_fixed(gg * dx, &bad), fy = to_fixed(gg * dy, &bad),
-                  fz = to_fixed(gg * dz, &bad);
-        atomicAdd(&Fr[si], (unsigned long long)fx);
-        atomicAdd(&Fr[Np + si], (unsigned long long)fy);
-        atomicAdd(&Fr[2 * Np + si], (unsigned long long)fz);
-        atomicAdd(&Fr[sj], (unsigned long long)(-fx));
-        atomicAdd(&Fr[Np + sj], (unsigned long long)(-fy));
-        atomicAdd(&Fr[2 * Np + sj], (unsigned long long)(-fz));
-    } else if (t < nbond + nang) {
-        int k = t - nbond;
-        int a = ang_atoms[3 * k], m = ang_atoms[3 * k + 1], c = ang_atoms[3 * k + 2];
-        int sa = S ? S[a] : a, sm = S ? S[m] : m, sc = S ? S[c] : c;
-        double v1[3], v2[3];
-        for (int d = 0; d < 3; d++) {
-            v1[d] = X[d * Np + a] - X[d * Np + m];
-            v2[d] = X[d * Np + c] - X[d * Np + m];
+    if (i < N) {
+        double f[3] = {0.0, 0.0, 0.0};
+        double xi[3] = {X[i], X[Np + i], X[2 * Np + i]};
+        for (int q = bstart[i], qe = bstart[i + 1]; q < qe; q++) {
+            int code = bcode[q], t = code >> 1, role = code & 1;
+            int j = bond_atoms[2 * t + (role ^ 1)];
+            double d[3];
+            for (int a = 0; a < 3; a++) d[a] = X[a * Np + j] - xi[a];
+            double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double dr = rr - bond_r0[t];
+            double gg = rr > 0.0 ? bond_k[t] * dr / rr : 0.0;
+            for (int a = 0; a < 3; a++) f[a] += gg * d[a];  // pulls i toward j when stretched
+            if (role == 0) e += 0.5 * bond_k[t] * dr * dr;
         }
-        double r1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-        double r2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
-        double cs = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (r1 * r2);
-        cs = fmin(fmax(cs, -1.0), 1.0);
-        double th = acos(cs);
-        double dth = th - ang_t0[k];
-        e = 0.5 * ang_k[k] * dth * dth;
-        double sn = fmax(sin(th), 1e-12);
-        double gg = ang_k[k] * dth / sn;
-        for (int d = 0; d < 3; d++) {
-            double fa = gg * (v2[d] / (r1 * r2) - cs * v1[d] / (r1 * r1));
-            double fc = gg * (v1[d] / (r1 * r2) - cs * v2[d] / (r2 * r2));
-            long long ia = to_fixed(fa, &bad), ic = to_fixed(fc, &bad);
-            atomicAdd(&Fr[d * Np + sa], (unsigned long long)ia);
-            atomicAdd(&Fr[d * Np + sc], (unsigned long long)ic);
-            atomicAdd(&Fr[d * Np + sm], (unsigned long long)(-(ia + ic)));
+        for (int q = astart[i], qe = astart[i + 1]; q < qe; q++) {
+            int code = acode[q], k = code >> 2, role = code & 3;
+            int ia = ang_atoms[3 * k], im = ang_atoms[3 * k + 1], ic = ang_atoms[3 * k + 2];
+            double v1[3], v2[3];
+            for (int a = 0; a < 3; a++) {
+                double xm = X[a * Np + im];
+                v1[a] = X[a * Np + ia] - xm;
+                v2[a] = X[a * Np + ic] - xm;
+            }
+            double r1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+            double r2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+            double cs = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (r1 * r2);
+            cs = fmin(fmax(cs, -1.0), 1.0);
+            double th = acos(cs);
+            double dth = th - ang_t0[k];
+            double gg = ang_k[k] * dth / fmax(sin(th), 1e-12);
+            if (role == 0) e += 0.5 * ang_k[k] * dth * dth;
+            for (int a = 0; a < 3; a++) {
+                double fa = gg * (v2[a] / (r1 * r2) - cs * v1[a] / (r1 * r1));
+                double fc = gg * (v1[a] / (r1 * r2) - cs * v2[a] / (r2 * r2));
+                f[a] += role == 0 ? fa : (role == 2 ? fc : -(fa + fc));
+            }
         }
+        bool bad = false;
+        double* Fr = Fb + (size_t)r * 3 * Np;
+        for (int a = 0; a < 3; a++) {
+            bad |= !(fabs(f[a]) < SDM_FIX_LIMIT);
+            Fr[a * Np + i] = f[a];
+        }
+        if (bad) atomicOr(&flags[r], 2);
     }
-    if (bad) atomicOr(&flags[r], 2);
     red[threadIdx.x] = e;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -2136,60 +2199,79 @@ __global__ __launch_bounds__(256) void k_terms(int Np, DevTerms T, const double*
 // displaced-state passes (FP64)
 // ---------------------------------------------------------------------------
 
-// candidate j-blocks: within rc of a displaced class's box at x or at x+d_k.
-__global__ __launch_bounds__(64) void k_ligcand(
+// candidate j-blocks: within rc of a displaced class's box at x or at x+d_k.  One
+// 256-thread workgroup per replica (4 waves share the class boxes' reductions and the
+// scan over the blocks, so the replica's serial chain is short); candidates are written
+// in ascending block order.
+constexpr int LIGCAND_THREADS = 256;
+__global__ __launch_bounds__(LIGCAND_THREADS) void k_ligcand(
     int Np, int NB, int L, int ncls, const int* __restrict__ lig, const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ x64,
     const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi, int maxc,
     int* __restrict__ cand, int* __restrict__ ncand, float list_r2, Geom g,
     int* __restrict__ flags) {
+    constexpr int NW = LIGCAND_THREADS / 64;
     __shared__ float4 blo[2 * SDM_MAX_CLASSES], bhi[2 * SDM_MAX_CLASSES];
-    int r = blockIdx.x, lane = threadIdx.x;
+    __shared__ double red[6][NW];
+    __shared__ int wcount[NW];
+    const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const double* X = x64 + (size_t)r * 3 * Np;
     for (int k = 1; k <= ncls; k++) {
-        double lx = 1e300, ly = 1e300, lz = 1e300, hx = -1e300, hy = -1e300, hz = -1e300;
-        for (int base = 0; base < L; base += 64) {
-            int li = base + lane;
-            if (li < L) {
-                int a = lig[li];
-                if (cls[a] == k) {
-                    lx = fmin(lx, X[a]);
-                    ly = fmin(ly, X[Np + a]);
-                    lz = fmin(lz, X[2 * Np + a]);
-                    hx = fmax(hx, X[a]);
-                    hy = fmax(hy, X[Np + a]);
-                    hz = fmax(hz, X[2 * Np + a]);
-                }
+        double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+        for (int li = t; li < L; li += LIGCAND_THREADS) {
+            int a = lig[li];
+            if (cls[a] != k) continue;
+            for (int d = 0; d < 3; d++) {
+                double x = X[d * Np + a];
+                v[d] = fmin(v[d], x);
+                v[3 + d] = fmax(v[3 + d], x);
             }
         }
-        lx = wave_min_d(lx); ly = wave_min_d(ly); lz = wave_min_d(lz);
-        hx = wave_max_d(hx); hy = wave_max_d(hy); hz = wave_max_d(hz);
-        if (lane == 0) {
-            const double* d = cdispl + 3 * k;
-            blo[2 * (k - 1)] = make_float4((float)lx, (float)ly, (float)lz, 0.f);
-            bhi[2 * (k - 1)] = make_float4((float)hx, (float)hy, (float)hz, 0.f);
-            blo[2 * (k - 1) + 1] = make_float4((float)(lx + d[0]), (float)(ly + d[1]), (float)(lz + d[2]), 0.f);
-            bhi[2 * (k - 1) + 1] = make_float4((float)(hx + d[0]), (float)(hy + d[1]), (float)(hz + d[2]), 0.f);
+        for (int d = 0; d < 3; d++) {
+            v[d] = wave_min_d(v[d]);
+            v[3 + d] = wave_max_d(v[3 + d]);
         }
+        if (lane == 0)
+            for (int q = 0; q < 6; q++) red[q][wv] = v[q];
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < NW; w++)
+                for (int d = 0; d < 3; d++) {
+                    v[d] = fmin(v[d], red[d][w]);
+                    v[3 + d] = fmax(v[3 + d], red[3 + d][w]);
+                }
+            const double* dk = cdispl + 3 * k;
+            blo[2 * (k - 1)] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.f);
+            bhi[2 * (k - 1)] = make_float4((float)v[3], (float)v[4], (float)v[5], 0.f);
+            blo[2 * (k - 1) + 1] = make_float4((float)(v[0] + dk[0]), (float)(v[1] + dk[1]),
+                                               (float)(v[2] + dk[2]), 0.f);
+            bhi[2 * (k - 1) + 1] = make_float4((float)(v[3] + dk[0]), (float)(v[4] + dk[1]),
+                                               (float)(v[5] + dk[2]), 0.f);
+        }
+        __syncthreads();  // red is reused by the next class
     }
-    __syncthreads();
     const float4* lo = bb_lo + (size_t)r * NB;
     const float4* hi = bb_hi + (size_t)r * NB;
     int* out = cand + (size_t)r * maxc;
-    int count = 0;
-    for (int base = 0; base < NB; base += 64) {
-        int jb = base + lane;
+    int count = 0;  // candidates before this round (uniform)
+    for (int base = 0; base < NB; base += LIGCAND_THREADS) {
+        int jb = base + t;
         bool ok = false;
         if (jb < NB) {
             float4 lj = lo[jb], hj = hi[jb];
             for (int k = 0; k < 2 * ncls; k++) ok = ok || (bbox_gap2(blo[k], bhi[k], lj, hj, g) < list_r2);
         }
         unsigned long long m = __ballot(ok);
-        int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wv] = __popcll(m);
+        __syncthreads();
+        int before = count;
+        for (int w = 0; w < wv; w++) before += wcount[w];
+        int pos = before + lane_rank(m);
         if (ok && pos < maxc) out[pos] = jb;
-        count += __popcll(m);
+        for (int w = 0; w < NW; w++) count += wcount[w];
+        __syncthreads();  // wcount is rewritten next round
     }
-    if (lane == 0) {
+    if (t == 0) {
         ncand[r] = min(count, maxc);
         if (count > maxc) atomicOr(&flags[r], 4);
     }
@@ -2291,20 +2373,6 @@ __device__ __forceinline__ Atom64 tile_atom(const AtomTile64& T, int t,
     return A;
 }
 
-// squared gap (FP64, minimum image of the centre difference) between point p and the box
-// [l, h]: a lower bound of the squared distance from p to every point of the box
-__device__ __forceinline__ double point_box_gap2(double px, double py, double pz, double lx,
-                                                 double ly, double lz, double hx, double hy,
-                                                 double hz, const Geom& g) {
-    double cx = min_image(px - 0.5 * (lx + hx), g.Lx, g.iLx);
-    double cy = min_image(py - 0.5 * (ly + hy), g.Ly, g.iLy);
-    double cz = min_image(pz - 0.5 * (lz + hz), g.Lz, g.iLz);
-    double gx = fmax(fabs(cx) - 0.5 * (hx - lx), 0.0);
-    double gy = fmax(fabs(cy) - 0.5 * (hy - ly), 0.0);
-    double gz = fmax(fabs(cz) - 0.5 * (hz - lz), 0.0);
-    return gx * gx + gy * gy + gz * gz;
-}
-
 // whether atom a has an exclusion partner that is displaced (want_displaced) or not.
 // Exclusions across the boundary of the displaced set are rare (a bond from the ligand
 // to the receptor), so only atoms that have one search their list per pair.
@@ -2318,92 +2386,143 @@ __device__ __forceinline__ bool excl_with(const int* __restrict__ ex_start,
     return any;
 }
 
-// lanes = displaced atoms (ligand block lb: entries lb_start[lb] .. + lb_count[lb] of
-// lig, all of one displacement class), j = undisplaced atoms of candidate block c.  DW
-// waves per workgroup share the 64 staged j atoms (wave w takes j = w, w + DW, ..), so a
-// workgroup's serial chain of FP64 pair evaluations is DW times shorter; their partials
-// are added in wave order (deterministic).
-constexpr int DELTA_WAVES = 4;
-template <int DW>
-__global__ __launch_bounds__(64 * DW) void k_deltaA(
+// The displaced-state pairs: every (displaced atom i, undisplaced atom j) within the
+// cutoff at x or at x+d, evaluated in FP64 at both geometries (delta_pair).  Two passes
+// evaluate each such pair once from each side, so that every sum has one owner lane and
+// a fixed order (no atomics):
+//   k_denv  lanes = the 64 slots of candidate block c.  An FP32 screen of the block's
+//           undisplaced atoms against every displaced atom (at x and at x+d, with a
+//           margin that covers FP32 rounding) gives each lane its 64-bit list of partners
+//           per ligand block, whose pairs it then evaluates in ascending order: F1-F0 on
+//           the undisplaced atom (dB).  The screen's ballot per displaced atom is the
+//           same list seen from the other side; it goes to mlig[r][c][li].
+//   k_dlig  lanes = the displaced atoms of ligand block lb, over DLIG_CB candidate blocks
+//           per workgroup: each lane evaluates the pairs of its mlig bits in ascending
+//           (block, slot) order: u and F1-F0 on the displaced atom, one partial per chunk
+//           of candidate blocks (dA), added in chunk order by k_deltaA_red.
+// Excluded pairs are left out of the lists (only atoms with an exclusion across the
+// displaced/undisplaced boundary search their exclusion list).
+constexpr float DSCREEN_MARGIN = 1e-3f;  // nm; FP32 position rounding is below 1e-6 nm
+constexpr int DLIG_CB = 8;               // candidate blocks per k_dlig workgroup
+
+__global__ __launch_bounds__(64) void k_denv(
     int N, int Np, int L, int LB, const int* __restrict__ lig,
     const int* __restrict__ lb_start, const int* __restrict__ lb_count,
-    const int* __restrict__ cls,
-    const double* __restrict__ cdispl, const double* __restrict__ q64,
-    const double* __restrict__ sig64, const double* __restrict__ eps64,
-    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
-    const int* __restrict__ ncand, int maxc, double4* __restrict__ dA, Geom g,
-    unsigned long long* __restrict__ pair_count) {
-    int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z;
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int* __restrict__ cls, const double* __restrict__ cdispl,
+    const double* __restrict__ q64, const double* __restrict__ sig64,
+    const double* __restrict__ eps64, const int* __restrict__ ex_start,
+    const int* __restrict__ ex_list, const double* __restrict__ x64,
+    const int* __restrict__ orig_s, const int* __restrict__ cand, const int* __restrict__ ncand,
+    int maxc, unsigned long long* __restrict__ mlig, double4* __restrict__ dB, Geom g) {
+    int c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
     if (c >= ncand[r]) return;  // uniform over the workgroup
+    const double* X = x64 + (size_t)r * 3 * Np;
+    int s = cand[(size_t)r * maxc + c] * 64 + lane;
+    bool v = s < N;
+    int b = v ? orig_s[(size_t)r * Np + s] : 0;
+    Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
+    bool env = v && J.cls == 0;
+    bool exJ = env && excl_with(ex_start, ex_list, cls, b, true);
+    const float jx = (float)J.x, jy = (float)J.y, jz = (float)J.z;
+    const float rs = (float)sqrt(g.rc2) + DSCREEN_MARGIN, rs2 = rs * rs;
+    const float bx = g.bx, by = g.by, bz = g.bz, ibx = g.ibx, iby = g.iby, ibz = g.ibz;
     __shared__ AtomTile64 T;
-    __shared__ double4 part[DW][64];
+    __shared__ float4 P0[64], P1[64];  // the ligand block at x and at x+d, FP32
+    unsigned long long* ML = mlig + ((size_t)r * maxc + c) * L;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int lb = 0; lb < LB; lb++) {
+        const int base = lb_start[lb], n = lb_count[lb];
+        __syncthreads();  // the previous ligand block has been read
+        if (lane < n) {
+            int a = lig[base + lane];
+            stage_atom(T, lane, X, Np, a, cls, q64, sig64, eps64);
+            const double* d = cdispl + 3 * cls[a];
+            double x = X[a], y = X[Np + a], z = X[2 * Np + a];
+            P0[lane] = make_float4((float)x, (float)y, (float)z, 0.f);
+            P1[lane] = make_float4((float)(x + d[0]), (float)(y + d[1]), (float)(z + d[2]), 0.f);
+        }
+        __syncthreads();
+        unsigned long long mine = 0ull, col = 0ull;
+        for (int k = 0; k < n; k++) {
+            float4 p0 = P0[k], p1 = P1[k];
+            float dx = jx - p0.x, dy = jy - p0.y, dz = jz - p0.z;
+            dx -= bx * rintf(dx * ibx);
+            dy -= by * rintf(dy * iby);
+            dz -= bz * rintf(dz * ibz);
+            float ex = jx - p1.x, ey = jy - p1.y, ez = jz - p1.z;
+            ex -= bx * rintf(ex * ibx);
+            ey -= by * rintf(ey * iby);
+            ez -= bz * rintf(ez * ibz);
+            bool in = env && (dx * dx + dy * dy + dz * dz < rs2 || ex * ex + ey * ey + ez * ez < rs2);
+            if (exJ && in) in = !excluded(ex_start, ex_list, b, T.idx[k]);
+            unsigned long long bal = __ballot(in);
+            if (lane == k) col = bal;
+            mine |= (unsigned long long)in << k;
+        }
+        if (lane < n) ML[base + lane] = col;
+        while (mine) {  // this lane's partners, ascending
+            int k = __ffsll((long long)mine) - 1;
+            mine &= mine - 1ull;
+            Atom64 I = tile_atom(T, k, cdispl);
+            int iny;
+            delta_pair(J, I, g, fx, fy, fz, &iny);
+        }
+    }
+    dB[((size_t)r * maxc + c) * 64 + lane] = make_double4(fx, fy, fz, 0.0);
+}
+
+__global__ __launch_bounds__(64) void k_dlig(
+    int N, int Np, int L, int LB, const int* __restrict__ lig,
+    const int* __restrict__ lb_start, const int* __restrict__ lb_count,
+    const int* __restrict__ cls, const double* __restrict__ cdispl,
+    const double* __restrict__ q64, const double* __restrict__ sig64,
+    const double* __restrict__ eps64, const double* __restrict__ x64,
+    const int* __restrict__ orig_s, const int* __restrict__ cand, const int* __restrict__ ncand,
+    int maxc, const unsigned long long* __restrict__ mlig, int nchunk,
+    double4* __restrict__ dA, Geom g, unsigned long long* __restrict__ pair_count) {
+    int ch = blockIdx.x, lb = blockIdx.y, r = blockIdx.z, lane = threadIdx.x;
+    const int nc = ncand[r], c0 = ch * DLIG_CB;
+    if (c0 >= nc) return;  // uniform; k_deltaA_red reads only the chunks below nc
     const double* X = x64 + (size_t)r * 3 * Np;
     int li = lb_start[lb] + lane;
     bool v = lane < lb_count[lb];
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
-    int jb = cand[(size_t)r * maxc + c];
-    int jbeg = jb * 64, jn = min(64, N - jbeg);
-    if ((int)threadIdx.x < jn)
-        stage_atom(T, threadIdx.x, X, Np, orig_s[(size_t)r * Np + jbeg + threadIdx.x], cls, q64,
-                   sig64, eps64);
-    bool exI = v && excl_with(ex_start, ex_list, cls, I.idx, false);
-    // boxes of this block of displaced atoms at x and at x+d (lanes past the block's
-    // count repeat its first atom): j atoms beyond the cutoff of both boxes
-    // have no pair inside it at either geometry and are skipped (their terms are zero)
-    double b0lx = wave_min_d(I.x), b0ly = wave_min_d(I.y), b0lz = wave_min_d(I.z);
-    double b0hx = wave_max_d(I.x), b0hy = wave_max_d(I.y), b0hz = wave_max_d(I.z);
-    double b1lx = wave_min_d(I.x + I.dx), b1ly = wave_min_d(I.y + I.dy),
-           b1lz = wave_min_d(I.z + I.dz);
-    double b1hx = wave_max_d(I.x + I.dx), b1hy = wave_max_d(I.y + I.dy),
-           b1hz = wave_max_d(I.z + I.dz);
-    const double cut2 = g.rc2 * (1.0 + 1e-9);
-    __syncthreads();
+    __shared__ AtomTile64 T;
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cnt = 0;
-    for (int jj = wv; jj < jn; jj += DW) {
-        if (T.cls[jj] != 0) continue;
-        Atom64 J = tile_atom(T, jj, cdispl);
-        if (point_box_gap2(J.x, J.y, J.z, b0lx, b0ly, b0lz, b0hx, b0hy, b0hz, g) >= cut2 &&
-            point_box_gap2(J.x + J.dx, J.y + J.dy, J.z + J.dz, b1lx, b1ly, b1lz, b1hx, b1hy,
-                           b1hz, g) >= cut2)
-            continue;  // wave-uniform
-        if (!v || (exI && excluded(ex_start, ex_list, I.idx, J.idx))) continue;
-        int iny;
-        du += delta_pair(I, J, g, fx, fy, fz, &iny);
-        cnt += iny;
+    for (int c = c0, ce = min(c0 + DLIG_CB, nc); c < ce; c++) {
+        unsigned long long m = v ? mlig[((size_t)r * maxc + c) * L + li] : 0ull;
+        if (__ballot(m != 0ull) == 0ull) continue;  // uniform
+        int s = cand[(size_t)r * maxc + c] * 64 + lane;
+        __syncthreads();  // the previous block has been read
+        if (s < N) stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + s], cls, q64, sig64, eps64);
+        __syncthreads();
+        while (m) {  // this lane's partners in block c, ascending slot
+            int k = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            Atom64 J = tile_atom(T, k, cdispl);
+            int iny;
+            du += delta_pair(I, J, g, fx, fy, fz, &iny);
+            cnt += iny;
+        }
     }
     if (pair_count) {
-        double s = wave_sum((double)cnt);
-        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(s + 0.5));
+        double sc = wave_sum((double)cnt);
+        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(sc + 0.5));
         return;
     }
-    part[wv][lane] = make_double4(fx, fy, fz, du);
-    __syncthreads();
-    if (wv == 0) {
-        double4 t = part[0][lane];
-        for (int k = 1; k < DW; k++) {
-            double4 p = part[k][lane];
-            t.x += p.x;
-            t.y += p.y;
-            t.z += p.z;
-            t.w += p.w;
-        }
-        dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane] = t;
-    }
+    dA[(((size_t)r * nchunk + ch) * LB + lb) * 64 + lane] = make_double4(fx, fy, fz, du);
 }
 
-// per displaced atom: candidate partials in order, plus pairs with displaced atoms of a
-// different class (D x D); writes dlig and the ligand block's u.
+// per displaced atom: the chunk partials of k_dlig in order, plus pairs with displaced
+// atoms of a different class (D x D); writes dlig and the ligand block's u.
 __global__ __launch_bounds__(64) void k_deltaA_red(
     int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ lb_start,
     const int* __restrict__ lb_count, const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ q64,
     const double* __restrict__ sig64, const double* __restrict__ eps64,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const double* __restrict__ x64, const int* __restrict__ ncand, int maxc,
+    const double* __restrict__ x64, const int* __restrict__ ncand, int nchunk,
     const double4* __restrict__ dA, double4* __restrict__ dlig, double* __restrict__ u_part,
     Geom g, unsigned long long* __restrict__ pair_count) {
     int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
@@ -2412,10 +2531,9 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     bool v = lane < lb_count[lb];
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-    int nc = ncand[r];
-#pragma unroll 8
-    for (int c = 0; c < nc; c++) {
-        double4 p = dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane];
+    int nch = (ncand[r] + DLIG_CB - 1) / DLIG_CB;
+    for (int ch = 0; ch < nch; ch++) {
+        double4 p = dA[(((size_t)r * nchunk + ch) * LB + lb) * 64 + lane];
         fx += p.x;
         fy += p.y;
         fz += p.z;
@@ -2444,74 +2562,6 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     if (v) dlig[(size_t)r * L + li] = make_double4(fx, fy, fz, du);
     double us = wave_sum(v ? du : 0.0);
     if (lane == 0) u_part[(size_t)r * LB + lb] = us;
-}
-
-// lanes = atoms of candidate block c (undisplaced only), loop over the displaced atoms
-// (DW waves per workgroup, wave w taking displaced atoms w, w + DW, .. of each staged
-// chunk; their partials are added in wave order).
-template <int DW>
-__global__ __launch_bounds__(64 * DW) void k_deltaB(
-    int N, int Np, int L, const int* __restrict__ lig, const int* __restrict__ cls,
-    const double* __restrict__ cdispl, const double* __restrict__ q64,
-    const double* __restrict__ sig64, const double* __restrict__ eps64,
-    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const double* __restrict__ x64, const int* __restrict__ orig_s, const int* __restrict__ cand,
-    const int* __restrict__ ncand, int maxc, double4* __restrict__ dB, Geom g) {
-    int c = blockIdx.x, r = blockIdx.y;
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (c >= ncand[r]) return;
-    const double* X = x64 + (size_t)r * 3 * Np;
-    int jb = cand[(size_t)r * maxc + c];
-    int s = jb * 64 + lane;
-    bool v = s < N;
-    int b = v ? orig_s[(size_t)r * Np + s] : 0;
-    Atom64 J = load_atom64(X, Np, b, cls, cdispl, q64, sig64, eps64);
-    bool env = v && J.cls == 0;
-    bool exJ = env && excl_with(ex_start, ex_list, cls, b, true);
-    // box of the block's undisplaced atoms at x and at x+d (empty if it has none):
-    // displaced atoms beyond the cutoff of both have no pair inside it with any of them
-    double blx = wave_min_d(env ? J.x : 1e300), bly = wave_min_d(env ? J.y : 1e300),
-           blz = wave_min_d(env ? J.z : 1e300);
-    double bhx = wave_max_d(env ? J.x : -1e300), bhy = wave_max_d(env ? J.y : -1e300),
-           bhz = wave_max_d(env ? J.z : -1e300);
-    double dlx = wave_min_d(env ? J.x + J.dx : 1e300), dly = wave_min_d(env ? J.y + J.dy : 1e300),
-           dlz = wave_min_d(env ? J.z + J.dz : 1e300);
-    double dhx = wave_max_d(env ? J.x + J.dx : -1e300), dhy = wave_max_d(env ? J.y + J.dy : -1e300),
-           dhz = wave_max_d(env ? J.z + J.dz : -1e300);
-    const bool any_env = __ballot(env) != 0ull;
-    const double cut2 = g.rc2 * (1.0 + 1e-9);
-    __shared__ AtomTile64 T;
-    __shared__ double4 part[DW][64];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int base = 0; base < L && any_env; base += 64) {
-        int n = min(64, L - base);
-        __syncthreads();  // the previous chunk has been read
-        if ((int)threadIdx.x < n)
-            stage_atom(T, threadIdx.x, X, Np, lig[base + threadIdx.x], cls, q64, sig64, eps64);
-        __syncthreads();
-        for (int k = wv; k < n; k += DW) {
-            Atom64 I = tile_atom(T, k, cdispl);
-            if (point_box_gap2(I.x, I.y, I.z, blx, bly, blz, bhx, bhy, bhz, g) >= cut2 &&
-                point_box_gap2(I.x + I.dx, I.y + I.dy, I.z + I.dz, dlx, dly, dlz, dhx, dhy,
-                               dhz, g) >= cut2)
-                continue;  // wave-uniform
-            if (!env || (exJ && excluded(ex_start, ex_list, b, I.idx))) continue;
-            int iny;
-            delta_pair(J, I, g, fx, fy, fz, &iny);
-        }
-    }
-    part[wv][lane] = make_double4(fx, fy, fz, 0.0);
-    __syncthreads();
-    if (wv == 0) {
-        double4 t = part[0][lane];
-        for (int k = 1; k < DW; k++) {
-            double4 p = part[k][lane];
-            t.x += p.x;
-            t.y += p.y;
-            t.z += p.z;
-        }
-        dB[((size_t)r * maxc + c) * 64 + lane] = t;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2721,10 +2771,13 @@ __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
     if (bad) atomicOr(&flags[r], 2);
 }
 
-// leapfrog Langevin (langevin.cl:7-69, double-precision branch).  Atoms with a
-// constraint (cflag) leave their displacement dt*v in dxs for k_constrain.
+// leapfrog Langevin (langevin.cl:7-69, double-precision branch).  The force is the
+// fixed-point accumulator (slot order) plus, when there are bonded terms, their FP64
+// forces Fb (atom order).  Atoms with a constraint (cflag) leave their displacement
+// dt*v in dxs for k_constrain.
 __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* __restrict__ invm,
                                                   const long long* __restrict__ F,
+                                                  const double* __restrict__ Fb,
                                                   double* __restrict__ x64, double* __restrict__ v64,
                                                   const DevRecord* __restrict__ rec,
                                                   const DevState* __restrict__ state,
@@ -2750,6 +2803,7 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double f = from_fixed(Fr[d * Np + slot]);
+        if (Fb) f += Fb[((size_t)r * 3 + d) * Np + i];
         double vel = vscale * V[d * Np + i] + fscale * im * f + noise * sim * gz[d];
         double delta = dt * vel;
         if (con) {
