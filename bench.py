@@ -15,8 +15,9 @@ work and the forces on every step.  Also reported: the same ladder with the pair
 on every step, the single-replica plain path (R=1) steps/s and ns/day, the batched/plain
 ratio, the 8-replica ladder of the 99,920-atom relative-binding system (two guests
 displaced by +-3 nm, rc 1.2 nm), the roofline of the pair kernel as the steps run it
-(forces only; FP32 VALU, 42 FLOP per in-cutoff pair) timed live with HIP events on the
-library stream, and the FP64 CPU baseline.
+(forces only; FP32 VALU, 42 FLOP per in-cutoff pair, and the 50-FLOP count that includes
+the energy terms) timed live with HIP events on the library stream, with the VALU
+counters of the committed rocprofv3 summary, and the FP64 CPU baseline.
 """
 import argparse
 import json
@@ -29,7 +30,8 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 50            # FMA = 2; see the roofline accounting in the design notes
 FLOP_PER_PAIR_FORCES = 42     # the same without the energy terms (LJ 2, RF 4, sums 2)
-FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 vector (published)
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 vector (published, packed FP32)
+FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (packed FP32 is not used)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -60,18 +62,23 @@ def parse():
     return ap.parse_args()
 
 
-def read_traffic():
-    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "k_pair_n3_config3_pmc.md")
+def read_profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1])
             except ValueError:
                 return None
     return None
+
+
+def read_traffic():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary."""
+    return read_profile_value("k_pair_n3_config3_pmc.md", "traffic_bytes_per_launch")
 
 
 def cpu_info():
@@ -239,7 +246,14 @@ def main():
     p0_b, _ = ctx.count_pairs()
     pairs = 0.5 * (float(p0_a.sum()) + float(p0_b.sum()))
     achieved = FLOP_PER_PAIR_FORCES * pairs / (pair_ms * 1e-3) / 1e12
+    achieved_50 = FLOP_PER_PAIR * pairs / (pair_ms * 1e-3) / 1e12
     traffic = read_traffic()
+    # VALU evidence from the committed counter summary (SQ_INSTS_VALU per launch, and the
+    # busy fraction derived from SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE)
+    vname = "k_pair_n3_config3_valu.md"
+    valu_busy = read_profile_value(vname, "valu_busy")
+    vinst = read_profile_value(vname, "valu_insts_per_launch")
+    valu_per_pair = 64.0 * vinst / pairs if vinst else None
 
     # ---- exactly K timed steps ----
     torch.cuda.synchronize()
@@ -338,6 +352,10 @@ def main():
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic, "pair_kernel_ms": pair_ms,
             "pairs_per_launch": pairs, "flop_per_pair": FLOP_PER_PAIR_FORCES,
+            "frac_50flop": achieved_50 / FP32_PEAK_TFLOPS,
+            "frac_of_nonpacked_peak": achieved / FP32_NONPACKED_TFLOPS,
+            "valu_busy": valu_busy,
+            "valu_lane_ops_per_in_cutoff_pair": valu_per_pair,
         },
         "cpu_baseline": cpu,
     }

@@ -160,12 +160,15 @@ typedef struct {
                                      Results differ only in rounding: each wave's i forces
                                      and pair energies are summed on their own before they
                                      are added up. */
-    int32_t side_fork;            /* where the bonded terms and the displaced-state chain
-                                     run: 0 on two side streams forked after the tile lists
-                                     (beside the pair kernel), 1 forked after the slot order
-                                     (beside the tile-list build as well), 2 on the main
-                                     stream between the tile lists and the pair kernel.
-                                     Same results. */
+    int32_t side_fork;            /* where the restraints, force-field terms and the
+                                     displaced-state chain run: 0 on two side streams
+                                     forked after the tile lists (beside the pair kernel),
+                                     1 forked after the slot order (beside the tile-list
+                                     build as well), 2 on the main stream between the tile
+                                     lists and the pair kernel.  With 0 and 1 the bonds
+                                     and angles start with the step on a side stream
+                                     (beside the spatial sort); with 2 they run with the
+                                     rest.  Same results. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */

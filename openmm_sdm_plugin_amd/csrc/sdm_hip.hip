@@ -78,7 +78,7 @@ struct sdm_context {
     // side streams of the step's fork/join: bonded terms and the displaced-state chain
     // run concurrently with the pair kernel
     hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork0 = nullptr, ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
     // 0: CutoffNonPeriodic (no images; the sort grid spans reg_lo .. reg_hi, the union of
     // the positions' bounding boxes over the replicas, widened by GRID_MARGIN)
@@ -146,7 +146,6 @@ struct sdm_context {
     // [R][maxc][L] 64-bit lists: the undisplaced slots of candidate block c that displaced
     // atom li pairs with (k_denv's screen, read by k_dlig)
     unsigned long long* mlig = nullptr;
-    int nchunk = 1;  // k_dlig chunks of candidate blocks
     DevState* state = nullptr;
     DevRecord* rec = nullptr;
     int* flags = nullptr;
@@ -506,16 +505,29 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
 // pair kernel.
 void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool energy,
                   hipEvent_t pe0 = nullptr, hipEvent_t pe1 = nullptr) {
-    // tuning.side_fork: the side work (bonded terms, restraints, force-field terms, the
-    // displaced-state chain) needs the slot order, block boxes and cleared accumulator,
-    // not the tile lists.  0: it forks onto the two side streams after the tile lists,
-    // 1: before them, 2: it runs on the main stream between the tile lists and the pair
-    // kernel (no fork; while the pair kernel fills the GPU, side kernels beside it wait
-    // for free wave slots and can outlast it)
+    // tuning.side_fork: the side work (restraints, force-field terms, the displaced-state
+    // chain) needs the slot order, block boxes and cleared accumulator, not the tile
+    // lists.  0: it forks onto the two side streams after the tile lists, 1: before them,
+    // 2: it runs on the main stream between the tile lists and the pair kernel (no fork;
+    // while the pair kernel fills the GPU, side kernels beside it wait for free wave
+    // slots and can outlast it).  The bonded terms need only the positions and write
+    // their own FP64 array: with a fork (0, 1) they start with the step, beside the sort.
     const int mode = c->tune.side_fork;
     hipStream_t s = c->stream;
     hipStream_t s1 = mode == 2 ? s : c->side[0], s2 = mode == 2 ? s : c->side[1];
     int N = c->N, Np = c->Np, R = c->R;
+    auto bonded = [&]() {
+        if (c->nbond + c->nang > 0)
+            k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(
+                N, Np, c->bstart, c->bcode, c->astart, c->acode, c->bond_atoms, c->bond_r0,
+                c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k, c->x64, c->Fb, c->e_bond,
+                c->nbw, c->flags);
+    };
+    if (mode != 2) {
+        (void)hipEventRecord(c->ev_fork0, s);
+        (void)hipStreamWaitEvent(s1, c->ev_fork0, 0);
+        bonded();
+    }
     enqueue_order(c, full, energy);
     if (mode != 1) enqueue_tiles(c, energy);
     if (mode != 2) {
@@ -524,11 +536,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
         (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
     }
     if (mode == 1) enqueue_tiles(c, energy);
-    if (c->nbond + c->nang > 0)
-        k_bonded<<<dim3(c->nbw, R), 256, 0, s1>>>(N, Np, c->bstart, c->bcode, c->astart, c->acode,
-                                                  c->bond_atoms, c->bond_r0, c->bond_k,
-                                                  c->ang_atoms, c->ang_t0, c->ang_k, c->x64,
-                                                  c->Fb, c->e_bond, c->nbw, c->flags);
+    if (mode == 2) bonded();
     if (c->ncen + c->nali > 0)
         k_restraints<<<dim3(c->ncen + c->nali, R), 64, 0, s1>>>(
             Np, c->ncen, c->ncen + c->nali, c->rcen, c->rali, c->ratom, c->rw, c->x64,
@@ -543,17 +551,17 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
         k_ligcand<<<R, LIGCAND_THREADS, 0, s2>>>(Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl, c->x64,
                                     c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand, c->list_r2,
                                     c->g, c->flags);
-        k_denv<<<dim3(c->maxc, R), 64, 0, s2>>>(
+        k_denv<DPAIR_WAVES><<<dim3(c->maxc, R), 64 * DPAIR_WAVES, 0, s2>>>(
             N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand, c->ncand,
             c->maxc, c->mlig, c->dB, c->g);
-        k_dlig<<<dim3(c->nchunk, c->LB, R), 64, 0, s2>>>(
+        k_dlig<DPAIR_WAVES><<<dim3(c->maxc, c->LB, R), 64 * DPAIR_WAVES, 0, s2>>>(
             N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig,
-            c->nchunk, c->dA, c->g, nullptr);
+            c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig, c->dA,
+            c->g, nullptr);
         k_deltaA_red<<<dim3(c->LB, R), 64, 0, s2>>>(
             Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->nchunk, c->dA,
+            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, nullptr);
     }
     if (pe0) (void)hipEventRecord(pe0, s);
@@ -806,6 +814,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork0, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess)
@@ -885,8 +894,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(e_pair, (size_t)R * NB * 8); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(e64, (size_t)R * NB * E64_SPLIT); A(rec64, RNp);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
-    c->nchunk = (c->maxc + DLIG_CB - 1) / DLIG_CB;
-    A(dA, (size_t)R * c->nchunk * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
+    A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
     A(mlig, (size_t)R * c->maxc * std::max(1, c->L));
     A(dB, (size_t)R * c->maxc * 64); A(state, R); A(rec, R); A(flags, R);
     A(uhist, (size_t)R * c->hist_cap); A(counters, 2 * (size_t)R); A(ke_out, R);
@@ -962,7 +970,7 @@ int sdm_destroy(sdm_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (void* p : c->allocs) (void)hipFree(p);
-    for (hipEvent_t e : {c->ev_fork, c->ev_join[0], c->ev_join[1]})
+    for (hipEvent_t e : {c->ev_fork0, c->ev_fork, c->ev_join[0], c->ev_join[1]})
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t st : {c->side[0], c->side[1]})
         if (st) (void)hipStreamDestroy(st);
@@ -1596,17 +1604,17 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
         k_ligcand<<<c->R, LIGCAND_THREADS, 0, s>>>(c->Np, c->NB, c->L, c->ncls, c->lig, c->cls, c->cdispl,
                                       c->x64, c->bb_lo, c->bb_hi, c->maxc, c->cand, c->ncand,
                                       c->list_r2, c->g, c->flags);
-        k_denv<<<dim3(c->maxc, c->R), 64, 0, s>>>(
+        k_denv<DPAIR_WAVES><<<dim3(c->maxc, c->R), 64 * DPAIR_WAVES, 0, s>>>(
             c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
             c->q64, c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->orig_s, c->cand,
             c->ncand, c->maxc, c->mlig, c->dB, c->g);
-        k_dlig<<<dim3(c->nchunk, c->LB, c->R), 64, 0, s>>>(
+        k_dlig<DPAIR_WAVES><<<dim3(c->maxc, c->LB, c->R), 64 * DPAIR_WAVES, 0, s>>>(
             c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
             c->q64, c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig,
-            c->nchunk, c->dA, c->g, c->counters + c->R);
+            c->dA, c->g, c->counters + c->R);
         k_deltaA_red<<<dim3(c->LB, c->R), 64, 0, s>>>(
             c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
-            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->nchunk, c->dA,
+            c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, c->counters + c->R);
     }
     CHK(check_launch());

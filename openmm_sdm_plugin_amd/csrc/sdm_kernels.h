@@ -361,23 +361,35 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
             float gz = fmaxf(fabsf(dz) - h.z - 0.5f * (hj.z - lj.z), 0.f);
             rel = __ballot(lane < 8 && h.x >= 0.f && gx * gx + gy * gy + gz * gz < list_r2);
         }
-        bool in = false;
-        while (rel) {
-            int q = __ffsll((long long)rel) - 1;
-            rel &= rel - 1ull;
-            float4 c = sub[q][0], h = sub[q][1];
-            float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
-            if (!one_image) {
-                dx -= g.bx * rintf(dx * g.ibx);
-                dy -= g.by * rintf(dy * g.iby);
-                dz -= g.bz * rintf(dz * g.ibz);
+        // the atom tests, with the per-pair image only where the tile needs it and the
+        // half-list rule only in the diagonal tile (both uniform: separate loops)
+        auto atom_in = [&](auto mic, auto diagc) {
+            constexpr bool MI = decltype(mic)::value, DIAG = decltype(diagc)::value;
+            bool in = false;
+            unsigned long long rq = rel;
+            while (rq) {
+                int q = __ffsll((long long)rq) - 1;
+                rq &= rq - 1ull;
+                float4 c = sub[q][0], h = sub[q][1];
+                float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+                if (MI) {
+                    dx -= g.bx * rintf(dx * g.ibx);
+                    dy -= g.by * rintf(dy * g.iby);
+                    dz -= g.bz * rintf(dz * g.ibz);
+                }
+                float gx = fmaxf(fabsf(dx) - h.x, 0.f), gy = fmaxf(fabsf(dy) - h.y, 0.f),
+                      gz = fmaxf(fabsf(dz) - h.z, 0.f);
+                bool inq = gx * gx + gy * gy + gz * gz < list_r2;
+                if (DIAG) inq = inq && s > ib * 64 + q * 8;
+                in = in || inq;
             }
-            float gx = fmaxf(fabsf(dx) - h.x, 0.f), gy = fmaxf(fabsf(dy) - h.y, 0.f),
-                  gz = fmaxf(fabsf(dz) - h.z, 0.f);
-            bool inq = gx * gx + gy * gy + gz * gz < list_r2;
-            if (half && jb2 == ib) inq = inq && s > ib * 64 + q * 8;
-            in = in || inq;
-        }
+            return in;
+        };
+        using T = std::true_type;
+        using Fa = std::false_type;
+        bool in;
+        if (half && jb2 == ib) in = one_image ? atom_in(Fa(), T()) : atom_in(T(), T());
+        else in = one_image ? atom_in(Fa(), Fa()) : atom_in(T(), Fa());
         return __ballot(in && (s < N));
     };
     int* out = tiles + ((size_t)r * NB + ib) * maxj;
@@ -406,7 +418,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
         int nc = ncand_s;
         base = base_s;  // every wave continues from wave 0's position (uniform loop)
         for (int c = wave; c < nc; c += NW) {
-            unsigned long long am = mask_of(cand[c]);
+            // the candidate is the same for the whole wave: scalar, so that its box
+            // loads and the uniform tests above compile to scalar code and branches
+            unsigned long long am = mask_of(__builtin_amdgcn_readfirstlane(cand[c]));
             if (lane == 0) cmk[c] = am;
         }
         __syncthreads();
@@ -2287,7 +2301,7 @@ __device__ __forceinline__ bool excluded(const int* __restrict__ ex_start,
 struct Atom64 {
     double x, y, z;      // position at x
     double dx, dy, dz;   // displacement
-    double qk, sig, eps; // q*sqrt(k_e), sigma, epsilon
+    double qk, sig, se;  // q*sqrt(k_e), sigma, sqrt(epsilon)
     int idx, cls;
 };
 
@@ -2307,7 +2321,7 @@ __device__ __forceinline__ Atom64 load_atom64(const double* __restrict__ X, int 
     A.dz = cdispl[3 * A.cls + 2];
     A.qk = q[a];
     A.sig = sig[a];
-    A.eps = eps[a];
+    A.se = sqrt(eps[a]);
     A.idx = a;
     return A;
 }
@@ -2318,7 +2332,7 @@ __device__ __forceinline__ double delta_pair(const Atom64& I, const Atom64& J, c
                                              double& fx, double& fy, double& fz, int* inside_y) {
     double qq = I.qk * J.qk;
     double sig = 0.5 * (I.sig + J.sig);
-    double eps4 = 4.0 * sqrt(I.eps * J.eps);
+    double eps4 = 4.0 * I.se * J.se;
     double ex, fr_x, ey, fr_y;
     double dx = min_image(I.x - J.x, g.Lx, g.iLx);
     double dy = min_image(I.y - J.y, g.Ly, g.iLy);
@@ -2337,38 +2351,42 @@ __device__ __forceinline__ double delta_pair(const Atom64& I, const Atom64& J, c
 // Up to 64 atoms staged in LDS by one wave (lane t copies atom t), so that the loops over
 // partner atoms read LDS broadcasts instead of chains of dependent global loads.
 struct AtomTile64 {
-    double x[64], y[64], z[64], q[64], sig[64], eps[64];
+    double x[64], y[64], z[64], dx[64], dy[64], dz[64], q[64], sig[64], se[64];
     int cls[64], idx[64];
 };
 
 __device__ __forceinline__ void stage_atom(AtomTile64& T, int t, const double* __restrict__ X,
                                            int Np, int a, const int* __restrict__ cls,
+                                           const double* __restrict__ cdispl,
                                            const double* __restrict__ q,
                                            const double* __restrict__ sig,
                                            const double* __restrict__ eps) {
+    int k = cls[a];
     T.x[t] = X[a];
     T.y[t] = X[Np + a];
     T.z[t] = X[2 * Np + a];
+    T.dx[t] = cdispl[3 * k];
+    T.dy[t] = cdispl[3 * k + 1];
+    T.dz[t] = cdispl[3 * k + 2];
     T.q[t] = q[a];
     T.sig[t] = sig[a];
-    T.eps[t] = eps[a];
-    T.cls[t] = cls[a];
+    T.se[t] = sqrt(eps[a]);
+    T.cls[t] = k;
     T.idx[t] = a;
 }
 
-__device__ __forceinline__ Atom64 tile_atom(const AtomTile64& T, int t,
-                                            const double* __restrict__ cdispl) {
+__device__ __forceinline__ Atom64 tile_atom(const AtomTile64& T, int t) {
     Atom64 A;
     A.x = T.x[t];
     A.y = T.y[t];
     A.z = T.z[t];
     A.cls = T.cls[t];
-    A.dx = cdispl[3 * A.cls];
-    A.dy = cdispl[3 * A.cls + 1];
-    A.dz = cdispl[3 * A.cls + 2];
+    A.dx = T.dx[t];
+    A.dy = T.dy[t];
+    A.dz = T.dz[t];
     A.qk = T.q[t];
     A.sig = T.sig[t];
-    A.eps = T.eps[t];
+    A.se = T.se[t];
     A.idx = T.idx[t];
     return A;
 }
@@ -2393,19 +2411,23 @@ __device__ __forceinline__ bool excl_with(const int* __restrict__ ex_start,
 //   k_denv  lanes = the 64 slots of candidate block c.  An FP32 screen of the block's
 //           undisplaced atoms against every displaced atom (at x and at x+d, with a
 //           margin that covers FP32 rounding) gives each lane its 64-bit list of partners
-//           per ligand block, whose pairs it then evaluates in ascending order: F1-F0 on
-//           the undisplaced atom (dB).  The screen's ballot per displaced atom is the
-//           same list seen from the other side; it goes to mlig[r][c][li].
-//   k_dlig  lanes = the displaced atoms of ligand block lb, over DLIG_CB candidate blocks
-//           per workgroup: each lane evaluates the pairs of its mlig bits in ascending
-//           (block, slot) order: u and F1-F0 on the displaced atom, one partial per chunk
-//           of candidate blocks (dA), added in chunk order by k_deltaA_red.
-// Excluded pairs are left out of the lists (only atoms with an exclusion across the
-// displaced/undisplaced boundary search their exclusion list).
+//           per ligand block, whose pairs it then evaluates: F1-F0 on the undisplaced
+//           atom (dB).  The screen's ballot per displaced atom is the same list seen
+//           from the other side; it goes to mlig[r][c][li].
+//   k_dlig  lanes = the displaced atoms of ligand block lb, one candidate block per
+//           workgroup: each lane evaluates the pairs of its mlig bits, giving u and F1-F0
+//           on the displaced atom, one partial per candidate block (dA), added in block
+//           order by k_deltaA_red.
+// A few blocks near the displaced atoms hold most pairs, so both spread a block's pairs
+// over DW waves (wave w takes the partners with index = w mod DW, ascending) and add
+// the waves' partials in wave order.  Excluded pairs are left out of the lists (only
+// atoms with an exclusion across the displaced/undisplaced boundary search their
+// exclusion list).
 constexpr float DSCREEN_MARGIN = 1e-3f;  // nm; FP32 position rounding is below 1e-6 nm
-constexpr int DLIG_CB = 8;               // candidate blocks per k_dlig workgroup
+constexpr int DPAIR_WAVES = 4;
 
-__global__ __launch_bounds__(64) void k_denv(
+template <int DW>
+__global__ __launch_bounds__(64 * DW) void k_denv(
     int N, int Np, int L, int LB, const int* __restrict__ lig,
     const int* __restrict__ lb_start, const int* __restrict__ lb_count,
     const int* __restrict__ cls, const double* __restrict__ cdispl,
@@ -2414,7 +2436,7 @@ __global__ __launch_bounds__(64) void k_denv(
     const int* __restrict__ ex_list, const double* __restrict__ x64,
     const int* __restrict__ orig_s, const int* __restrict__ cand, const int* __restrict__ ncand,
     int maxc, unsigned long long* __restrict__ mlig, double4* __restrict__ dB, Geom g) {
-    int c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    int c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (c >= ncand[r]) return;  // uniform over the workgroup
     const double* X = x64 + (size_t)r * 3 * Np;
     int s = cand[(size_t)r * maxc + c] * 64 + lane;
@@ -2425,82 +2447,105 @@ __global__ __launch_bounds__(64) void k_denv(
     bool exJ = env && excl_with(ex_start, ex_list, cls, b, true);
     const float jx = (float)J.x, jy = (float)J.y, jz = (float)J.z;
     const float rs = (float)sqrt(g.rc2) + DSCREEN_MARGIN, rs2 = rs * rs;
-    const float bx = g.bx, by = g.by, bz = g.bz, ibx = g.ibx, iby = g.iby, ibz = g.ibz;
     __shared__ AtomTile64 T;
     __shared__ float4 P0[64], P1[64];  // the ligand block at x and at x+d, FP32
+    __shared__ double4 part[DW][64];
     unsigned long long* ML = mlig + ((size_t)r * maxc + c) * L;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int lb = 0; lb < LB; lb++) {
         const int base = lb_start[lb], n = lb_count[lb];
         __syncthreads();  // the previous ligand block has been read
-        if (lane < n) {
-            int a = lig[base + lane];
-            stage_atom(T, lane, X, Np, a, cls, q64, sig64, eps64);
+        if ((int)threadIdx.x < n) {
+            int a = lig[base + threadIdx.x];
+            stage_atom(T, threadIdx.x, X, Np, a, cls, cdispl, q64, sig64, eps64);
             const double* d = cdispl + 3 * cls[a];
             double x = X[a], y = X[Np + a], z = X[2 * Np + a];
-            P0[lane] = make_float4((float)x, (float)y, (float)z, 0.f);
-            P1[lane] = make_float4((float)(x + d[0]), (float)(y + d[1]), (float)(z + d[2]), 0.f);
+            P0[threadIdx.x] = make_float4((float)x, (float)y, (float)z, 0.f);
+            P1[threadIdx.x] =
+                make_float4((float)(x + d[0]), (float)(y + d[1]), (float)(z + d[2]), 0.f);
         }
         __syncthreads();
         unsigned long long mine = 0ull, col = 0ull;
-        for (int k = 0; k < n; k++) {
+        for (int k = wv; k < n; k += DW) {
             float4 p0 = P0[k], p1 = P1[k];
             float dx = jx - p0.x, dy = jy - p0.y, dz = jz - p0.z;
-            dx -= bx * rintf(dx * ibx);
-            dy -= by * rintf(dy * iby);
-            dz -= bz * rintf(dz * ibz);
+            dx -= g.bx * rintf(dx * g.ibx);
+            dy -= g.by * rintf(dy * g.iby);
+            dz -= g.bz * rintf(dz * g.ibz);
             float ex = jx - p1.x, ey = jy - p1.y, ez = jz - p1.z;
-            ex -= bx * rintf(ex * ibx);
-            ey -= by * rintf(ey * iby);
-            ez -= bz * rintf(ez * ibz);
+            ex -= g.bx * rintf(ex * g.ibx);
+            ey -= g.by * rintf(ey * g.iby);
+            ez -= g.bz * rintf(ez * g.ibz);
             bool in = env && (dx * dx + dy * dy + dz * dz < rs2 || ex * ex + ey * ey + ez * ez < rs2);
             if (exJ && in) in = !excluded(ex_start, ex_list, b, T.idx[k]);
             unsigned long long bal = __ballot(in);
             if (lane == k) col = bal;
             mine |= (unsigned long long)in << k;
         }
-        if (lane < n) ML[base + lane] = col;
-        while (mine) {  // this lane's partners, ascending
+        if (lane < n && lane % DW == wv) ML[base + lane] = col;
+        while (mine) {  // this lane's partners of this wave, ascending
             int k = __ffsll((long long)mine) - 1;
             mine &= mine - 1ull;
-            Atom64 I = tile_atom(T, k, cdispl);
+            Atom64 I = tile_atom(T, k);
             int iny;
             delta_pair(J, I, g, fx, fy, fz, &iny);
         }
     }
-    dB[((size_t)r * maxc + c) * 64 + lane] = make_double4(fx, fy, fz, 0.0);
+    part[wv][lane] = make_double4(fx, fy, fz, 0.0);
+    __syncthreads();
+    if (wv == 0) {
+        double4 t = part[0][lane];
+        for (int w = 1; w < DW; w++) {
+            t.x += part[w][lane].x;
+            t.y += part[w][lane].y;
+            t.z += part[w][lane].z;
+        }
+        dB[((size_t)r * maxc + c) * 64 + lane] = t;
+    }
 }
 
-__global__ __launch_bounds__(64) void k_dlig(
+template <int DW>
+__global__ __launch_bounds__(64 * DW) void k_dlig(
     int N, int Np, int L, int LB, const int* __restrict__ lig,
     const int* __restrict__ lb_start, const int* __restrict__ lb_count,
     const int* __restrict__ cls, const double* __restrict__ cdispl,
     const double* __restrict__ q64, const double* __restrict__ sig64,
     const double* __restrict__ eps64, const double* __restrict__ x64,
     const int* __restrict__ orig_s, const int* __restrict__ cand, const int* __restrict__ ncand,
-    int maxc, const unsigned long long* __restrict__ mlig, int nchunk,
-    double4* __restrict__ dA, Geom g, unsigned long long* __restrict__ pair_count) {
-    int ch = blockIdx.x, lb = blockIdx.y, r = blockIdx.z, lane = threadIdx.x;
-    const int nc = ncand[r], c0 = ch * DLIG_CB;
-    if (c0 >= nc) return;  // uniform; k_deltaA_red reads only the chunks below nc
+    int maxc, const unsigned long long* __restrict__ mlig, double4* __restrict__ dA, Geom g,
+    unsigned long long* __restrict__ pair_count) {
+    int c = blockIdx.x, lb = blockIdx.y, r = blockIdx.z;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (c >= ncand[r]) return;  // uniform; k_deltaA_red reads only the blocks below ncand
     const double* X = x64 + (size_t)r * 3 * Np;
     int li = lb_start[lb] + lane;
     bool v = lane < lb_count[lb];
-    Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
+    // this wave's partners: slots = wv mod DW
+    unsigned long long sel = 0ull;
+    for (int k = wv; k < 64; k += DW) sel |= 1ull << k;
+    unsigned long long m = v ? (mlig[((size_t)r * maxc + c) * L + li] & sel) : 0ull;
     __shared__ AtomTile64 T;
+    __shared__ double4 part[DW][64];
+    __shared__ int any_s;
+    if (threadIdx.x == 0) any_s = 0;
+    __syncthreads();
+    const bool wany = __ballot(m != 0ull) != 0ull;
+    if (lane == 0 && wany) any_s = 1;
+    __syncthreads();
+    const bool any = any_s != 0;  // uniform over the workgroup
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cnt = 0;
-    for (int c = c0, ce = min(c0 + DLIG_CB, nc); c < ce; c++) {
-        unsigned long long m = v ? mlig[((size_t)r * maxc + c) * L + li] : 0ull;
-        if (__ballot(m != 0ull) == 0ull) continue;  // uniform
+    if (any) {
         int s = cand[(size_t)r * maxc + c] * 64 + lane;
-        __syncthreads();  // the previous block has been read
-        if (s < N) stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + s], cls, q64, sig64, eps64);
+        if (wv == 0 && s < N)
+            stage_atom(T, lane, X, Np, orig_s[(size_t)r * Np + s], cls, cdispl, q64, sig64, eps64);
+        Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64,
+                               eps64);
         __syncthreads();
-        while (m) {  // this lane's partners in block c, ascending slot
+        while (m) {  // this lane's partners of this wave, ascending slot
             int k = __ffsll((long long)m) - 1;
             m &= m - 1ull;
-            Atom64 J = tile_atom(T, k, cdispl);
+            Atom64 J = tile_atom(T, k);
             int iny;
             du += delta_pair(I, J, g, fx, fy, fz, &iny);
             cnt += iny;
@@ -2508,13 +2553,24 @@ __global__ __launch_bounds__(64) void k_dlig(
     }
     if (pair_count) {
         double sc = wave_sum((double)cnt);
-        if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(sc + 0.5));
+        if (lane == 0 && any) atomicAdd(&pair_count[r], (unsigned long long)(sc + 0.5));
         return;
     }
-    dA[(((size_t)r * nchunk + ch) * LB + lb) * 64 + lane] = make_double4(fx, fy, fz, du);
+    part[wv][lane] = make_double4(fx, fy, fz, du);
+    __syncthreads();
+    if (wv == 0) {
+        double4 t = part[0][lane];
+        for (int w = 1; w < DW; w++) {
+            t.x += part[w][lane].x;
+            t.y += part[w][lane].y;
+            t.z += part[w][lane].z;
+            t.w += part[w][lane].w;
+        }
+        dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane] = t;
+    }
 }
 
-// per displaced atom: the chunk partials of k_dlig in order, plus pairs with displaced
+// per displaced atom: the block partials of k_dlig in order, plus pairs with displaced
 // atoms of a different class (D x D); writes dlig and the ligand block's u.
 __global__ __launch_bounds__(64) void k_deltaA_red(
     int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ lb_start,
@@ -2522,7 +2578,7 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     const double* __restrict__ cdispl, const double* __restrict__ q64,
     const double* __restrict__ sig64, const double* __restrict__ eps64,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const double* __restrict__ x64, const int* __restrict__ ncand, int nchunk,
+    const double* __restrict__ x64, const int* __restrict__ ncand, int maxc,
     const double4* __restrict__ dA, double4* __restrict__ dlig, double* __restrict__ u_part,
     Geom g, unsigned long long* __restrict__ pair_count) {
     int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
@@ -2531,9 +2587,10 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     bool v = lane < lb_count[lb];
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-    int nch = (ncand[r] + DLIG_CB - 1) / DLIG_CB;
-    for (int ch = 0; ch < nch; ch++) {
-        double4 p = dA[(((size_t)r * nchunk + ch) * LB + lb) * 64 + lane];
+    const int nc = ncand[r];
+#pragma unroll 8
+    for (int c = 0; c < nc; c++) {
+        double4 p = dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane];
         fx += p.x;
         fy += p.y;
         fz += p.z;
@@ -2544,10 +2601,10 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     for (int base = 0; base < L; base += 64) {
         int n = min(64, L - base);
         __syncthreads();  // the previous chunk has been read
-        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, q64, sig64, eps64);
+        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, cdispl, q64, sig64, eps64);
         __syncthreads();
         for (int k = 0; k < n; k++) {
-            Atom64 J = tile_atom(T, k, cdispl);
+            Atom64 J = tile_atom(T, k);
             if (!v || J.cls == I.cls || excluded(ex_start, ex_list, I.idx, J.idx)) continue;
             int iny;
             du += 0.5 * delta_pair(I, J, g, fx, fy, fz, &iny);
@@ -3022,8 +3079,8 @@ __global__ __launch_bounds__(256) void k_u_frames(
         load_frame_atom(X, j, J.x, J.y, J.z);
         I.dx = cdispl[3 * ci]; I.dy = cdispl[3 * ci + 1]; I.dz = cdispl[3 * ci + 2];
         J.dx = cdispl[3 * cj]; J.dy = cdispl[3 * cj + 1]; J.dz = cdispl[3 * cj + 2];
-        I.qk = q64[i]; I.sig = sig64[i]; I.eps = eps64[i];
-        J.qk = q64[j]; J.sig = sig64[j]; J.eps = eps64[j];
+        I.qk = q64[i]; I.sig = sig64[i]; I.se = sqrt(eps64[i]);
+        J.qk = q64[j]; J.sig = sig64[j]; J.se = sqrt(eps64[j]);
         double fx = 0, fy = 0, fz = 0;
         int iny;
         acc += delta_pair(I, J, g, fx, fy, fz, &iny);
