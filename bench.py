@@ -81,6 +81,11 @@ def read_traffic():
     return read_profile_value("k_pair_n3_config3_pmc.md", "traffic_bytes_per_launch")
 
 
+def note(msg):
+    """Progress of the legs on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_info():
     """Core count (nproc) and model name (lscpu) of the machine the CPU legs run on."""
     import subprocess
@@ -151,9 +156,10 @@ def cpu_baseline(n_steps, warmup=10, runs=3, omp_steps=2):
 # ladder and the 8-replica 99,920-atom ladder (used by the profiling modes, which do not
 # autotune)
 LADDER_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=4, pair_waves_per_group=1,
-                     cell_cells_per_dim=20, sort_interval=20, use_graph=0, energy_every_step=0)
-LARGE_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=2, pair_waves_per_group=1,
-                    cell_cells_per_dim=33, sort_interval=20, energy_every_step=0)
+                     cell_cells_per_dim=20, sort_interval=20, use_graph=0, energy_every_step=0,
+                     side_fork=1)
+LARGE_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=4, pair_waves_per_group=1,
+                    cell_cells_per_dim=33, sort_interval=20, energy_every_step=0, side_fork=1)
 
 
 def large_ladder_context(systems, _native):
@@ -233,6 +239,7 @@ def main():
                           "calibration_bytes": src.numel() * 4, "calibration_sum": tot}))
         return
     tuning = ctx.autotune(40) if not a.no_autotune else ctx.get_tuning()
+    note(f"ladder R={R} tuned: {tuning}")
 
     # ---- batched ladder: warm-up ----
     ctx.step(a.warmup)
@@ -255,6 +262,7 @@ def main():
     vinst = read_profile_value(vname, "valu_insts_per_launch")
     valu_per_pair = 64.0 * vinst / pairs if vinst else None
 
+    note(f"pair kernel {pair_ms:.4f} ms per launch")
     # ---- exactly K timed steps ----
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -266,6 +274,7 @@ def main():
     flags = [x["flags"] for x in e]
     ok = all(np.isfinite(x["pot_energy"]) for x in e) and not any(flags)
 
+    note(f"timed ladder: {agg:.0f} steps/s")
     # ---- the same with the pair energies on every step ----
     ctx.set_tuning(energy_every_step=1)
     ctx.step(a.warmup)
@@ -279,6 +288,7 @@ def main():
     ok = ok and not any(x["flags"] for x in e_all)
     ctx.set_tuning(energy_every_step=0)
 
+    note(f"energies every step: {agg_all:.0f} steps/s")
     # ---- plain single-replica path (R = 1) ----
     one = _native.NativeContext(frames[0], 1)
     one.set_params(0, params[R // 2] if R > 1 else params[0])
@@ -294,6 +304,7 @@ def main():
     single = a.single_steps / (time.perf_counter() - t1)
     del one
 
+    note(f"single replica: {single:.0f} steps/s")
     # ---- the large relative-binding ladder: 8 replicas of the 99,920-atom system (two
     # guests displaced by +-3 nm, rc 1.2 nm), all starting from one frame ----
     large = None
@@ -316,9 +327,12 @@ def main():
                  "tuning": tuning5}
         del big
 
+    if large:
+        note(f"large ladder: {large['aggregate_steps_per_s']:.0f} steps/s")
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_steps)
+        note(f"cpu baseline: {cpu['value']:.2f} {cpu['unit']}")
 
     out = {
         "metric": "aggregate MD steps/s across batched lambda-replicas at 23k atoms; "

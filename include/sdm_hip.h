@@ -163,8 +163,8 @@ typedef struct {
     int32_t side_fork;            /* where the restraints, force-field terms and the
                                      displaced-state chain run: 0 on two side streams
                                      forked after the tile lists (beside the pair kernel),
-                                     1 forked after the slot order (beside the tile-list
-                                     build as well), 2 on the main stream between the tile
+                                     1 (the default) forked after the slot order (beside
+                                     the tile-list build as well), 2 on the main stream between the tile
                                      lists and the pair kernel.  With 0 and 1 the bonds
                                      and angles start with the step on a side stream
                                      (beside the spatial sort); with 2 they run with the

@@ -159,7 +159,7 @@ struct sdm_context {
     size_t sel_cap = 0;
     // this device's `scratch` (selection staging of the allocation-free adjoint)
     int32_t* scratch_sel = nullptr;
-    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4, 0};
+    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4, 1};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
     // captured steps: [with a full sort][with the pair energies]
