@@ -371,10 +371,10 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 
 /* Launch-configuration search (project-written autotuner): times each candidate (pair
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
-   for `steps` steps, re-times the three fastest for 3 x `steps` steps, each with and
-   without graph replay and with each placement of the side work (side_fork), and
-   keeps the fastest.  The state the context had before the
-   call is restored. */
+   for `steps` steps, then times the three fastest, each with and without graph replay
+   and with each placement of the side work (side_fork), for 3 x `steps` steps in two
+   rounds, and keeps the one with the smallest total.  The state the context had before
+   the call is restored. */
 int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
 int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);
 int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);

@@ -876,7 +876,9 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int max_ex,
     long long* __restrict__ F, double* __restrict__ e_pair, int* __restrict__ flags,
     unsigned long long* __restrict__ pair_count, Geom g) {
-    __shared__ float4 tile_a[WPW][128], tile_b[WPW][128];
+    // two tile buffers of 64 slot records (2 float4 each) plus 8 zero records: a chunk's
+    // entries past the tile's mask read there or the tile's own finite records
+    __shared__ float4 tile_a[WPW][144], tile_b[WPW][144];
     __shared__ float facc[WPW][3][64];
     extern __shared__ int exs_dyn[];
     int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -904,6 +906,10 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         for (int k = 0; k < nex; k++) exs[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
     }
     float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
+    if (lane < 16) {
+        tile_a[wv][128 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        tile_b[wv][128 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     double ax = 0.0, ay = 0.0, az = 0.0, aqr2 = 0.0, aq = 0.0;
     PairAcc acc;
     acc.e = 0.0;
@@ -960,6 +966,19 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
             ps.z -= sz;
         }
         const bool ex = __ballot(em != 0ull) != 0ull;
+        // the mask's records to the front of the buffer in mask order (entry e = the e-th
+        // set bit), so that a chunk reads its W records at fixed offsets from one address.
+        // Every lane reads its own record before any lane writes (one wave: its LDS
+        // accesses are carried out in order); the records past the mask stay finite.
+        {
+            float4* bw = B ? tile_b[wv] : tile_a[wv];
+            const float4 r0 = bw[2 * lane], r1 = bw[2 * lane + 1];
+            if ((am0 >> lane) & 1ull) {
+                const int rank = lane_rank(am0);
+                bw[2 * rank] = r0;
+                bw[2 * rank + 1] = r1;
+            }
+        }
         acc.fx = acc.fy = acc.fz = acc.sqr2 = acc.sq = 0.f;
         // all j atoms of the tile, 8 at a time; MI: per-pair minimum image, EX: the tile
         // holds excluded pairs (both wave-uniform, compile-time inside the loop).  ent:
@@ -976,13 +995,13 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
 #pragma unroll
                 for (int k = 0; k < W; k++) jj[k] = next_bit(am);
                 float v[3 * W];
+                const float4* cb = buf + 2 * ent;  // the chunk's records, in entry order
 #pragma unroll
                 for (int k = 0; k < W; k++) {
-                    int j = jj[k] < 0 ? jj[0] : jj[k];
-                    float4 pj = buf[2 * j];
-                    float2 qj = *(const float2*)&buf[2 * j + 1];
-                    bool incl = vi && jj[k] >= 0;
-                    if (EX) incl = incl && !((em >> j) & 1ull);
+                    float4 pj = cb[2 * k];
+                    float2 qj = *(const float2*)&cb[2 * k + 1];
+                    bool incl = vi && k < nvalid;
+                    if (EX) incl = incl && !((em >> (jj[k] & 63)) & 1ull);
                     float dx, dy, dz;
                     float fr = pair_masked<ENERGY>(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
                     v[3 * k] = -fr * dx;
@@ -1574,6 +1593,11 @@ __global__ __launch_bounds__(64 * WPW) void k_pair_grp(
 // ---------------------------------------------------------------------------
 constexpr int E64_SPLIT = 4;  // waves (parts) per i-block of the FP64 energy kernel
 
+// Orders the LDS accesses of a one-wave workgroup around this point.  A wave's LDS
+// instructions are carried out in issue order, so a compiler barrier is enough: unlike
+// __syncthreads() it does not wait for the wave's global loads in flight.
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
 __global__ __launch_bounds__(64) void k_energy64(
     int R, int N, int Np, int NB, int maxj, int split, int half, const float4* __restrict__ arec,
     const Rec64* __restrict__ rec, const int* __restrict__ orig_s,
@@ -1645,9 +1669,8 @@ __global__ __launch_bounds__(64) void k_energy64(
     // screen of one j atom against the wave's i atoms, queueing the pairs inside the
     // widened cutoff.  MI: per-pair minimum image (otherwise the tile's shift is in ps);
     // CHK: some lane is padding or has an exclusion partner in the tile
-    auto screen = [&](int j, float4 ps, unsigned long long em, auto mic, auto chk) {
+    auto screen = [&](int j, float4 pj, float4 ps, unsigned long long em, auto mic, auto chk) {
         constexpr bool MI = decltype(mic)::value, CHK = decltype(chk)::value;
-        const float4 pj = jpos[j];
         float dx = ps.x - pj.x, dy = ps.y - pj.y, dz = ps.z - pj.z;
         if (MI) {
             dx -= g.bx * rintf(dx * g.ibx);
@@ -1660,21 +1683,48 @@ __global__ __launch_bounds__(64) void k_energy64(
         if (in) queue[(tail + (unsigned)lane_rank(m)) & 127u] = (unsigned short)(lane | (j << 6));
         tail += (unsigned)__popcll(m);
         if (tail - head >= 64u) {
-            __syncthreads();  // the queue entries are written
+            lds_order();  // the queue entries are written
             evaluate(64u);
-            __syncthreads();  // and read before they are overwritten
+            lds_order();  // and read before they are overwritten
         }
     };
+    // the j records of the wave's next tile are loaded into registers while the current
+    // tile is screened and evaluated, and stored to LDS when its turn comes
+    // (plain vector values, not a struct, so that they stay in registers).  This lane's
+    // i record is used here first, so that the loop's waits for it are settled before
+    // the first prefetch is in flight (a wait inside the loop would wait for that too)
+    asm volatile("" ::"v"(pi.x), "v"(pi.y), "v"(pi.z), "v"(pi.w));
+    double2 na = make_double2(0.0, 0.0), nb = na, nc = na;  // a record is 3 x 16 bytes
+    float4 npos = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (part < nt) {
+        const int js = tl[part] * 64 + lane;
+        if (js < N) {
+            const double2* src = (const double2*)(RC + js);
+            na = src[0];
+            nb = src[1];
+            nc = src[2];
+            npos = A[2 * js];
+        }
+    }
     for (int t = part; t < nt; t += split) {
         const int jb = tl[t];
-        if (!half && jb < ib) continue;  // full lists: each block pair once
         unsigned long long am = tmk[t];
-        const int js = jb * 64 + lane;
-        __syncthreads();  // the previous tile's records are no longer read
-        if (js < N) {
-            jrec[lane] = RC[js];
-            jpos[lane] = A[2 * js];
+        lds_order();  // the previous tile's records are no longer read
+        if (jb * 64 + lane < N) {
+            jrec[lane] = Rec64{na.x, na.y, nb.x, nb.y, nc.x, nc.y};
+            jpos[lane] = npos;
         }
+        if (t + split < nt) {
+            const int js = tl[t + split] * 64 + lane;
+            if (js < N) {
+                const double2* src = (const double2*)(RC + js);
+                na = src[0];
+                nb = src[1];
+                nc = src[2];
+                npos = A[2 * js];
+            }
+        }
+        if (!half && jb < ib) continue;  // full lists: each block pair once
         unsigned long long em = 0;
         for (int k = 0; k < nex; k++) {
             int sl = exs64[k * 64 + lane];
@@ -1704,17 +1754,32 @@ __global__ __launch_bounds__(64) void k_energy64(
         } else {
             sx = sy = sz = 0.0;
         }
-        __syncthreads();
+        lds_order();  // this tile's records are stored
+        // the mask's j atoms four at a time: their four position reads are in flight
+        // together instead of one LDS round trip per atom
+        auto screen4 = [&](auto mic, auto chk) {
+            while (am) {
+                int jq[4];
+                float4 pq[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) jq[k] = next_bit(am);
+#pragma unroll
+                for (int k = 0; k < 4; k++) pq[k] = jpos[jq[k] & 63];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (jq[k] >= 0) screen(jq[k], pq[k], ps, em, mic, chk);
+            }
+        };
         using T = std::true_type;
         using Fa = std::false_type;
         if (one_image) {
-            if (chk) while (am) screen(next_bit(am), ps, em, Fa(), T());
-            else while (am) screen(next_bit(am), ps, em, Fa(), Fa());
+            if (chk) screen4(Fa(), T());
+            else screen4(Fa(), Fa());
         } else {
-            if (chk) while (am) screen(next_bit(am), ps, em, T(), T());
-            else while (am) screen(next_bit(am), ps, em, T(), Fa());
+            if (chk) screen4(T(), T());
+            else screen4(T(), Fa());
         }
-        __syncthreads();
+        lds_order();
         evaluate(tail - head);  // the tile's remaining pairs (its j records change next)
     }
     e = wave_sum(e);

@@ -76,6 +76,33 @@ def check_components(s, params=None, kernel="grp16"):
     return ctx
 
 
+@pytest.mark.parametrize("kernel", ["block_n3_split4", "block_full"])
+def test_exclusions_across_the_displaced_set(hip_device, kernel):
+    """Exclusions between displaced and undisplaced atoms (a guest bonded to its host):
+    the first two guest atoms each get exclusions with their three nearest water atoms
+    (0.24-0.37 nm away, inside the cutoff), so that the displaced-state screen has to
+    leave those pairs out at both geometries and the pair kernels at x."""
+    import dataclasses
+    s = systems.make_small(frame=1)
+    x = np.asarray(s.positions)
+    lig = np.nonzero(np.any(np.asarray(s.displacement) != 0.0, axis=1))[0]
+    env = np.setdiff1d(np.arange(s.n), lig)
+    box = np.asarray(s.box)
+    extra = []
+    for a in lig[:2]:
+        d = x[env] - x[a]
+        d -= box * np.round(d / box)
+        near = env[np.argsort((d * d).sum(1))[:3]]
+        extra += [(int(a), int(b)) for b in near]
+    ex = np.concatenate([np.asarray(s.exclusions).reshape(-1, 2), np.asarray(extra)], axis=0)
+    s2 = dataclasses.replace(s, exclusions=ex.astype(np.int32), name="small-cross-excl")
+    check_components(s2, kernel=kernel)
+    # the exclusions change the result: without them u differs
+    _, e_ref, _ = gpu_components(s, kernel=kernel)
+    _, e_x, _ = gpu_components(s2, kernel=kernel)
+    assert abs(e_x["u"] - e_ref["u"]) > 1e-6 * max(1.0, abs(e_ref["u"]))
+
+
 @pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("frame", [0, 1, 2])
 def test_small_components(hip_device, frame, kernel):
