@@ -164,11 +164,13 @@ typedef struct {
                                      displaced-state chain run: 0 on two side streams
                                      forked after the tile lists (beside the pair kernel),
                                      1 (the default) forked after the slot order (beside
-                                     the tile-list build as well), 2 on the main stream between the tile
-                                     lists and the pair kernel.  With 0 and 1 the bonds
-                                     and angles start with the step on a side stream
-                                     (beside the spatial sort); with 2 they run with the
-                                     rest.  Same results. */
+                                     the tile-list build as well), 2 on the main stream
+                                     between the tile lists and the pair kernel, 3 as 1
+                                     but with the displaced-state chain joined back
+                                     before the pair kernel (beside the tile lists only).
+                                     With 0, 1 and 3 the bonds and angles start with the
+                                     step on a side stream (beside the spatial sort);
+                                     with 2 they run with the rest.  Same results. */
 } sdm_tuning;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -372,9 +374,9 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 /* Launch-configuration search (project-written autotuner): times each candidate (pair
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
    for `steps` steps, then times the three fastest, each with and without graph replay
-   and with each placement of the side work (side_fork), for 3 x `steps` steps in two
-   rounds, and keeps the one with the smallest total.  The state the context had before
-   the call is restored. */
+   and with each of the four placements of the side work (side_fork), for 3 x `steps`
+   steps in two rounds, and keeps the one with the smallest total.  The state the
+   context had before the call is restored. */
 int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
 int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);
 int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);

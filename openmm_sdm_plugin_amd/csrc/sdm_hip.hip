@@ -510,8 +510,10 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     // lists.  0: it forks onto the two side streams after the tile lists, 1: before them,
     // 2: it runs on the main stream between the tile lists and the pair kernel (no fork;
     // while the pair kernel fills the GPU, side kernels beside it wait for free wave
-    // slots and can outlast it).  The bonded terms need only the positions and write
-    // their own FP64 array: with a fork (0, 1) they start with the step, beside the sort.
+    // slots and can outlast it), 3: as 1, but the displaced-state chain joins the main
+    // stream before the pair kernel (it runs beside the tile lists only).  The bonded
+    // terms need only the positions and write their own FP64 array: with a fork (0, 1,
+    // 3) they start with the step, beside the sort.
     const int mode = c->tune.side_fork;
     hipStream_t s = c->stream;
     hipStream_t s1 = mode == 2 ? s : c->side[0], s2 = mode == 2 ? s : c->side[1];
@@ -529,13 +531,14 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
         bonded();
     }
     enqueue_order(c, full, energy);
-    if (mode != 1) enqueue_tiles(c, energy);
+    const bool tiles_after_fork = mode == 1 || mode == 3;
+    if (!tiles_after_fork) enqueue_tiles(c, energy);
     if (mode != 2) {
         (void)hipEventRecord(c->ev_fork, s);
         (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
         (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
     }
-    if (mode == 1) enqueue_tiles(c, energy);
+    if (tiles_after_fork) enqueue_tiles(c, energy);
     if (mode == 2) bonded();
     if (c->ncen + c->nali > 0)
         k_restraints<<<dim3(c->ncen + c->nali, R), 64, 0, s1>>>(
@@ -564,6 +567,10 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, nullptr);
     }
+    if (mode == 3) {
+        (void)hipEventRecord(c->ev_join[1], s2);
+        (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
+    }
     if (pe0) (void)hipEventRecord(pe0, s);
     // forces in FP32; the pair energies, when asked for, in FP64 by their own kernel
     enqueue_pair(c, false, false);
@@ -571,8 +578,10 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     if (energy) enqueue_energy64(c);
     if (mode != 2) {
         (void)hipEventRecord(c->ev_join[0], s1);
-        (void)hipEventRecord(c->ev_join[1], s2);
         (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
+    }
+    if (mode == 0 || mode == 1) {
+        (void)hipEventRecord(c->ev_join[1], s2);
         (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     }
     k_scalar<<<R, 256, 0, s>>>(c->NB * E64_SPLIT, c->nbw, c->LB, c->e64, c->e_bond, c->u_part, c->state,
@@ -1687,8 +1696,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4 or 8");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
-    if (t->side_fork < 0 || t->side_fork > 2)
-        return fail(SDM_ERR_INVALID, "side_fork must be 0, 1 or 2");
+    if (t->side_fork < 0 || t->side_fork > 3)
+        return fail(SDM_ERR_INVALID, "side_fork must be 0, 1, 2 or 3");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -1775,7 +1784,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
                      [](const auto& p, const auto& q) { return p.first < q.first; });
     std::vector<sdm_tuning> finalists;
     for (size_t k = 0; k < timed.size() && k < 3; k++)
-        for (int variant = 0; variant < 6; variant++) {
+        for (int variant = 0; variant < 8; variant++) {
             sdm_tuning t = timed[k].second;
             t.use_graph = variant & 1;
             t.side_fork = variant >> 1;
