@@ -562,7 +562,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             N, Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig, c->dA,
             c->g, nullptr);
-        k_deltaA_red<<<dim3(c->LB, R), 64, 0, s2>>>(
+        k_deltaA_red<DRED_WAVES><<<dim3(c->LB, R), 64 * DRED_WAVES, 0, s2>>>(
             Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, nullptr);
@@ -900,7 +900,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
-    A(e_pair, (size_t)R * NB * 8); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
+    A(e_pair, (size_t)R * NB * 16); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(e64, (size_t)R * NB * E64_SPLIT); A(rec64, RNp);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
     A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
@@ -1621,7 +1621,7 @@ int sdm_count_pairs(sdm_context* c, int64_t* p0, int64_t* pdelta) {
             c->N, c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl,
             c->q64, c->sig64, c->eps64, c->x64, c->orig_s, c->cand, c->ncand, c->maxc, c->mlig,
             c->dA, c->g, c->counters + c->R);
-        k_deltaA_red<<<dim3(c->LB, c->R), 64, 0, s>>>(
+        k_deltaA_red<DRED_WAVES><<<dim3(c->LB, c->R), 64 * DRED_WAVES, 0, s>>>(
             c->Np, c->L, c->LB, c->lig, c->lb_start, c->lb_count, c->cls, c->cdispl, c->q64,
             c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, c->counters + c->R);
@@ -1692,8 +1692,9 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "energy_every_step must be 0 or 1");
     if (t->pair_newton3 != 0 && t->pair_newton3 != 1)
         return fail(SDM_ERR_INVALID, "pair_newton3 must be 0 or 1");
-    if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8)
-        return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4 or 8");
+    if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8 &&
+        t->pair_split != 16)
+        return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4, 8 or 16");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
     if (t->side_fork < 0 || t->side_fork > 3)
@@ -1749,10 +1750,10 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
         CHK(upload(c, c->flags, fl.data(), c->R));
         return SDM_OK;
     };
-    // pair kernels: block waves without Newton-3, with Newton-3 and 1, 2, 4 or 8 waves
+    // pair kernels: block waves without Newton-3, with Newton-3 and 1, 2, 4, 8 or 16 waves
     // per i-block, grouped waves of 16 and of 8
     const int kernels[][3] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 1, 4}, {0, 1, 8},
-                              {16, 1, 1}, {8, 1, 1}};
+                              {0, 1, 16}, {16, 1, 1}, {8, 1, 1}};
     std::vector<std::pair<double, sdm_tuning>> timed;
     int rc = SDM_OK;
     for (double side : cell_sides)

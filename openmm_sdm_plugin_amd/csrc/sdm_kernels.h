@@ -2637,7 +2637,8 @@ __global__ __launch_bounds__(64 * DW) void k_dlig(
 
 // per displaced atom: the block partials of k_dlig in order, plus pairs with displaced
 // atoms of a different class (D x D); writes dlig and the ligand block's u.
-__global__ __launch_bounds__(64) void k_deltaA_red(
+template <int RW>
+__global__ __launch_bounds__(64 * RW) void k_deltaA_red(
     int Np, int L, int LB, const int* __restrict__ lig, const int* __restrict__ lb_start,
     const int* __restrict__ lb_count, const int* __restrict__ cls,
     const double* __restrict__ cdispl, const double* __restrict__ q64,
@@ -2646,15 +2647,17 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
     const double* __restrict__ x64, const int* __restrict__ ncand, int maxc,
     const double4* __restrict__ dA, double4* __restrict__ dlig, double* __restrict__ u_part,
     Geom g, unsigned long long* __restrict__ pair_count) {
-    int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    // RW waves: wave w adds the partials of blocks w, w + RW, .. and the D x D pairs of
+    // displaced atoms w, w + RW, ..; the waves' sums are added in wave order
+    int lb = blockIdx.x, r = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double* X = x64 + (size_t)r * 3 * Np;
     int li = lb_start[lb] + lane;
     bool v = lane < lb_count[lb];
     Atom64 I = load_atom64(X, Np, v ? lig[li] : lig[lb_start[lb]], cls, cdispl, q64, sig64, eps64);
     double du = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     const int nc = ncand[r];
-#pragma unroll 8
-    for (int c = 0; c < nc; c++) {
+#pragma unroll 4
+    for (int c = wv; c < nc; c += RW) {
         double4 p = dA[(((size_t)r * maxc + c) * LB + lb) * 64 + lane];
         fx += p.x;
         fy += p.y;
@@ -2662,13 +2665,16 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
         du += p.w;
     }
     __shared__ AtomTile64 T;
+    __shared__ double4 part[RW][64];
     int cnt = 0;
     for (int base = 0; base < L; base += 64) {
         int n = min(64, L - base);
         __syncthreads();  // the previous chunk has been read
-        if (lane < n) stage_atom(T, lane, X, Np, lig[base + lane], cls, cdispl, q64, sig64, eps64);
+        if ((int)threadIdx.x < n)
+            stage_atom(T, threadIdx.x, X, Np, lig[base + threadIdx.x], cls, cdispl, q64, sig64,
+                       eps64);
         __syncthreads();
-        for (int k = 0; k < n; k++) {
+        for (int k = wv; k < n; k += RW) {
             Atom64 J = tile_atom(T, k);
             if (!v || J.cls == I.cls || excluded(ex_start, ex_list, I.idx, J.idx)) continue;
             int iny;
@@ -2681,10 +2687,22 @@ __global__ __launch_bounds__(64) void k_deltaA_red(
         if (lane == 0) atomicAdd(&pair_count[r], (unsigned long long)(s + 0.5));
         return;
     }
-    if (v) dlig[(size_t)r * L + li] = make_double4(fx, fy, fz, du);
-    double us = wave_sum(v ? du : 0.0);
-    if (lane == 0) u_part[(size_t)r * LB + lb] = us;
+    part[wv][lane] = make_double4(fx, fy, fz, du);
+    __syncthreads();
+    if (wv == 0) {
+        double4 t = part[0][lane];
+        for (int w = 1; w < RW; w++) {
+            t.x += part[w][lane].x;
+            t.y += part[w][lane].y;
+            t.z += part[w][lane].z;
+            t.w += part[w][lane].w;
+        }
+        if (v) dlig[(size_t)r * L + li] = t;
+        double us = wave_sum(v ? t.w : 0.0);
+        if (lane == 0) u_part[(size_t)r * LB + lb] = us;
+    }
 }
+constexpr int DRED_WAVES = 4;
 
 // ---------------------------------------------------------------------------
 // scalar stage (one workgroup per replica): energies, SoftCoreF, bias, sp, schedule.

@@ -15,10 +15,11 @@ pytestmark = pytest.mark.gpu
 E_TOL = 1e-5
 F_TOL = 1e-4
 
-# pair kernels: block waves with / without Newton-3 (with Newton-3 also 4 waves per
-# i-block), grouped waves of 16 and 8 atoms
+# pair kernels: block waves with / without Newton-3 (with Newton-3 also 4 and 16 waves
+# per i-block), grouped waves of 16 and 8 atoms
 KERNELS = {"block_n3": dict(pair_group=0, pair_newton3=1, pair_split=1),
            "block_n3_split4": dict(pair_group=0, pair_newton3=1, pair_split=4),
+           "block_n3_split16": dict(pair_group=0, pair_newton3=1, pair_split=16),
            "block_full": dict(pair_group=0, pair_newton3=0),
            "grp16": dict(pair_group=16, pair_newton3=1),
            "grp8": dict(pair_group=8, pair_newton3=1)}
@@ -406,7 +407,7 @@ def test_energy_on_last_step_of_call(hip_device):
     assert a7["U0"] != a8["U0"]  # the single-step call evaluated its own energies
 
 
-@pytest.mark.parametrize("use_graph,split", [(0, 1), (0, 4), (1, 8)])
+@pytest.mark.parametrize("use_graph,split", [(0, 1), (0, 4), (1, 8), (1, 16)])
 def test_forces_only_steps_match_energy_steps_on_ladder(hip_device, use_graph, split):
     """The forces-only pair kernel that steps use between energy evaluations gives the
     forces of the energy variant bit for bit on a full-size batch (22 replicas of the
