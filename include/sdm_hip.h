@@ -153,10 +153,11 @@ typedef struct {
                                      1: on every step.  The pair energies are FP64 sums
                                      over the FP64 positions (their own kernel); the
                                      forces are FP32 pair arithmetic. */
-    int32_t pair_split;           /* block Newton-3 kernel only: 1, 2, 4 (the default), 8
-                                     or 16 waves per i-block, each taking every pair_split-th
-                                     tile, so that short tile lists do not leave the GPU
-                                     waiting on long ones and small batches fill it.
+    int32_t pair_split;           /* block Newton-3 kernel only: 1, 2, 4 (the default), 8,
+                                     16 or 32 waves per i-block, each taking every
+                                     pair_split-th tile, so that short tile lists do not
+                                     leave the GPU waiting on long ones and small batches
+                                     fill it.
                                      Results differ only in rounding: each wave's i forces
                                      and pair energies are summed on their own before they
                                      are added up. */

@@ -900,7 +900,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
-    A(e_pair, (size_t)R * NB * 16); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
+    A(e_pair, (size_t)R * NB * 32); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(e64, (size_t)R * NB * E64_SPLIT); A(rec64, RNp);
     A(cand, (size_t)R * c->maxc); A(ncand, R);
     A(dA, (size_t)R * c->maxc * c->LB * 64); A(dlig, (size_t)R * std::max(1, c->L));
@@ -1693,8 +1693,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
     if (t->pair_newton3 != 0 && t->pair_newton3 != 1)
         return fail(SDM_ERR_INVALID, "pair_newton3 must be 0 or 1");
     if (t->pair_split != 1 && t->pair_split != 2 && t->pair_split != 4 && t->pair_split != 8 &&
-        t->pair_split != 16)
-        return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4, 8 or 16");
+        t->pair_split != 16 && t->pair_split != 32)
+        return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4, 8, 16 or 32");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
     if (t->side_fork < 0 || t->side_fork > 3)
@@ -1750,10 +1750,10 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
         CHK(upload(c, c->flags, fl.data(), c->R));
         return SDM_OK;
     };
-    // pair kernels: block waves without Newton-3, with Newton-3 and 1, 2, 4, 8 or 16 waves
+    // pair kernels: block waves without Newton-3, with Newton-3 and 1, 2, 4, 8, 16 or 32 waves
     // per i-block, grouped waves of 16 and of 8
     const int kernels[][3] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 2}, {0, 1, 4}, {0, 1, 8},
-                              {0, 1, 16}, {16, 1, 1}, {8, 1, 1}};
+                              {0, 1, 16}, {0, 1, 32}, {16, 1, 1}, {8, 1, 1}};
     std::vector<std::pair<double, sdm_tuning>> timed;
     int rc = SDM_OK;
     for (double side : cell_sides)

@@ -407,7 +407,7 @@ def test_energy_on_last_step_of_call(hip_device):
     assert a7["U0"] != a8["U0"]  # the single-step call evaluated its own energies
 
 
-@pytest.mark.parametrize("use_graph,split", [(0, 1), (0, 4), (1, 8), (1, 16)])
+@pytest.mark.parametrize("use_graph,split", [(0, 1), (0, 4), (1, 8), (1, 16), (0, 32)])
 def test_forces_only_steps_match_energy_steps_on_ladder(hip_device, use_graph, split):
     """The forces-only pair kernel that steps use between energy evaluations gives the
     forces of the energy variant bit for bit on a full-size batch (22 replicas of the
