@@ -899,12 +899,28 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
     float4 qi4 = A[2 * islot + 1];
     float2 qi = make_float2(qi4.x, qi4.y);
     int nex = 0;
+    // the range of j-blocks that hold an exclusion partner of one of the wave's i atoms
+    // (bonded partners are close: mostly this block and its neighbours); tiles outside
+    // it skip the exclusion test (wave-uniform)
+    int exlo = NB, exhi = -1;
     if (vi) {
         int ai = orig_s[(size_t)r * Np + islot];
         int b0 = ex_start[ai];
         nex = ex_start[ai + 1] - b0;
-        for (int k = 0; k < nex; k++) exs[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+        for (int k = 0; k < nex; k++) {
+            const int sl = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+            exs[k * 64 + lane] = sl;
+            exlo = min(exlo, sl >> 6);
+            exhi = max(exhi, sl >> 6);
+        }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        exlo = min(exlo, __shfl_xor(exlo, o, 64));
+        exhi = max(exhi, __shfl_xor(exhi, o, 64));
+    }
+    exlo = __builtin_amdgcn_readfirstlane(exlo);
+    exhi = __builtin_amdgcn_readfirstlane(exhi);
     float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
     if (lane < 16) {
         tile_a[wv][128 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -942,9 +958,11 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
         const unsigned long long am0 = am;
         if (t + split < nt) stage(tl[t + split], std::integral_constant<int, 1 - B>());
         unsigned long long em = 0;
-        for (int k = 0; k < nex; k++) {
-            int sl = exs[k * 64 + lane];
-            if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+        if (jb >= exlo && jb <= exhi) {
+            for (int k = 0; k < nex; k++) {
+                int sl = exs[k * 64 + lane];
+                if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+            }
         }
         if (jb == ib) em |= (2ull << lane) - 1ull;
         // one periodic shift for the whole tile when it is exact for every pair within
@@ -1624,12 +1642,25 @@ __global__ __launch_bounds__(64) void k_energy64(
     const float4 pi = A[2 * islot];
     irec[lane] = RC[vi ? islot : ib * 64];
     int nex = 0;
+    int exlo = NB, exhi = -1;  // j-blocks holding exclusion partners (as in k_pair_n3)
     if (vi) {
         int ai = orig_s[(size_t)r * Np + islot];
         int b0 = ex_start[ai];
         nex = ex_start[ai + 1] - b0;
-        for (int k = 0; k < nex; k++) exs64[k * 64 + lane] = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+        for (int k = 0; k < nex; k++) {
+            const int sl = slot_of[(size_t)r * Np + ex_list[b0 + k]];
+            exs64[k * 64 + lane] = sl;
+            exlo = min(exlo, sl >> 6);
+            exhi = max(exhi, sl >> 6);
+        }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        exlo = min(exlo, __shfl_xor(exlo, o, 64));
+        exhi = max(exhi, __shfl_xor(exhi, o, 64));
+    }
+    exlo = __builtin_amdgcn_readfirstlane(exlo);
+    exhi = __builtin_amdgcn_readfirstlane(exhi);
     const float rcs = sqrtf(g.rc2f) + 1e-4f;
     const float rc2s = rcs * rcs;
     const float4 bli = bb_lo[(size_t)r * NB + ib], bhi = bb_hi[(size_t)r * NB + ib];
@@ -1726,9 +1757,11 @@ __global__ __launch_bounds__(64) void k_energy64(
         }
         if (!half && jb < ib) continue;  // full lists: each block pair once
         unsigned long long em = 0;
-        for (int k = 0; k < nex; k++) {
-            int sl = exs64[k * 64 + lane];
-            if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+        if (jb >= exlo && jb <= exhi) {
+            for (int k = 0; k < nex; k++) {
+                int sl = exs64[k * 64 + lane];
+                if ((sl >> 6) == jb) em |= 1ull << (sl & 63);
+            }
         }
         if (jb == ib) em |= (2ull << lane) - 1ull;  // the diagonal tile: j > i only
         const bool chk = !full_block || __ballot(em != 0ull) != 0ull;
