@@ -52,6 +52,12 @@ def parse():
                     help="profiling mode: N eager steps of the 22-replica ladder, no other legs")
     ap.add_argument("--side-fork", type=int, default=None,
                     help="profiling modes: placement of the side work (sdm_tuning.side_fork)")
+    ap.add_argument("--list-skin", type=float, default=None,
+                    help="profiling modes: Verlet skin of the tile lists in nm "
+                         "(sdm_tuning.list_skin)")
+    ap.add_argument("--equilibrate", type=int, default=0,
+                    help="profiling modes: steps run (from captured graphs) before the "
+                         "profiled ones, so that the lattice-built start has relaxed")
     ap.add_argument("--profile-large", type=int, default=0,
                     help="profiling mode: N eager steps of the 8-replica 99,920-atom ladder "
                          "(its benchmark tuning, kernels launched one by one), no other legs")
@@ -189,9 +195,13 @@ def main():
     torch.cuda.set_device(0)
     if a.profile_large:
         big = large_ladder_context(systems, _native)
-        big.set_tuning(**LARGE_TUNING, use_graph=0)
+        big.set_tuning(**LARGE_TUNING, use_graph=1)
         if a.side_fork is not None:
             big.set_tuning(side_fork=a.side_fork)
+        if a.list_skin is not None:
+            big.set_tuning(list_skin=a.list_skin)
+        big.step(a.equilibrate)
+        big.set_tuning(use_graph=0)
         big.step(a.profile_large)
         e5 = big.energies()
         print(json.dumps({"profile_large_steps": a.profile_large, "step": e5[0]["step"],
@@ -220,9 +230,13 @@ def main():
             print(json.dumps({"pair_kernel_ms_events": ms, "tuning": ctx.get_tuning()}))
         else:
             # one launch per kernel (no graph) so a profiler attributes every dispatch
-            ctx.set_tuning(**LADDER_TUNING)
+            ctx.set_tuning(**dict(LADDER_TUNING, use_graph=1))
             if a.side_fork is not None:
                 ctx.set_tuning(side_fork=a.side_fork)
+            if a.list_skin is not None:
+                ctx.set_tuning(list_skin=a.list_skin)
+            ctx.step(a.equilibrate)
+            ctx.set_tuning(use_graph=0)
             ctx.step(a.profile)
         e = ctx.energies()
         # calibration for the HBM-traffic counters: a fill that writes 64 MiB and a sum

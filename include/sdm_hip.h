@@ -172,7 +172,20 @@ typedef struct {
                                      With 0, 1 and 3 the bonds and angles start with the
                                      step on a side stream (beside the spatial sort);
                                      with 2 they run with the rest.  Same results. */
+    float list_skin;              /* Verlet skin of the tile lists in nm, 0 to
+                                     SDM_MAX_LIST_SKIN (block pair kernels only; 0, the
+                                     default, builds them on every evaluation).  Lists
+                                     are built for r_c + list_skin and kept while every
+                                     atom of the replica has moved less than list_skin / 2
+                                     since, decided on the device each step; the pair
+                                     kernel still applies the exact cutoff.  With
+                                     pair_split 1 results are the same bits as with 0;
+                                     otherwise they differ in rounding (the waves' tile
+                                     shares change). */
 } sdm_tuning;
+
+/* largest sdm_tuning.list_skin (nm); the tile capacity is sized for it */
+#define SDM_MAX_LIST_SKIN 0.1
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* sdm_last_error(void);
@@ -376,8 +389,9 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
    for `steps` steps, then times the three fastest, each with and without graph replay
    and with each of the four placements of the side work (side_fork), for 3 x `steps`
-   steps in two rounds, and keeps the one with the smallest total.  The state the
-   context had before the call is restored. */
+   steps in two rounds, and keeps the one with the smallest total; with a block pair
+   kernel it then times that one with list_skin 0, 0.01, 0.02, 0.03 and 0.04 nm the same
+   way.  The state the context had before the call is restored. */
 int sdm_autotune(sdm_context* ctx, int32_t steps, sdm_tuning* chosen);
 int sdm_get_tuning(sdm_context* ctx, sdm_tuning* t);
 int sdm_set_tuning(sdm_context* ctx, const sdm_tuning* t);

@@ -75,7 +75,7 @@ class sdm_tuning(C.Structure):
                 ("use_graph", C.c_int32), ("pair_newton3", C.c_int32),
                 ("pair_group", C.c_int32), ("sort_interval", C.c_int32),
                 ("energy_every_step", C.c_int32), ("pair_split", C.c_int32),
-                ("side_fork", C.c_int32)]
+                ("side_fork", C.c_int32), ("list_skin", C.c_float)]
 
 
 class sdm_centroid_restraint(C.Structure):
@@ -468,7 +468,7 @@ class NativeContext:
         t = sdm_tuning()
         check(self._lib.sdm_get_tuning(self.h, C.byref(t)))
         for k, v in kw.items():
-            setattr(t, k, int(v))
+            setattr(t, k, float(v) if k == "list_skin" else int(v))
         check(self._lib.sdm_set_tuning(self.h, C.byref(t)))
 
     def du_dparams_frames_noalloc(self, frames_ptr, nframes, sel, grad_ptr, u_ptr):

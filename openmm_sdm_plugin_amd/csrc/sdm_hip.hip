@@ -116,6 +116,10 @@ struct sdm_context {
     float4 *bb_lo = nullptr, *bb_hi = nullptr;
     int *tiles = nullptr, *ntiles = nullptr;
     unsigned long long* tmask = nullptr;
+    float4* xb = nullptr;   // [R][Np] slot coordinates when the tile lists were built
+    float* dmax = nullptr;  // [R][DISP_STRIDE] (first float used) largest squared
+                            // displacement of the replica's atoms since then (k_refresh),
+                            // or stale after a sort (k_bbox)
     long long* F = nullptr;
     double *e_pair = nullptr, *e_bond = nullptr, *u_part = nullptr;
     double* e64 = nullptr;  // [R][NB * E64_SPLIT] FP64 pair-energy slots
@@ -159,7 +163,7 @@ struct sdm_context {
     size_t sel_cap = 0;
     // this device's `scratch` (selection staging of the allocation-free adjoint)
     int32_t* scratch_sel = nullptr;
-    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4, 1};
+    sdm_tuning tune{1, 0, 1, 1, 0, 20, 0, 4, 1, 0.f};
     // steps evaluated in the current slot order (0: stale, the next evaluation sorts)
     int since_sort = 0;
     // captured steps: [with a full sort][with the pair energies]
@@ -344,6 +348,12 @@ int setup_cells(sdm_context* c, int per_dim) {
 
 dim3 g2(int n, int per, int R) { return dim3((unsigned)((n + per - 1) / per), (unsigned)R); }
 
+// Tile lists are kept over steps (tuning.list_skin > 0) with the block pair kernels only:
+// there every slot refresh is followed by a k_tiles launch, which rebuilds after a sort,
+// so the build coordinates (xb) always belong to the lists in use.  The grouped kernel
+// builds lists on energy evaluations only; they are built every time.
+bool lists_kept(const sdm_context* c) { return c->tune.list_skin > 0.f && c->tune.pair_group == 0; }
+
 // slot order (a full spatial sort, or the last sort's order refreshed) and block boxes
 // for every replica
 // energy: also the slot-ordered FP64 records of the energy kernel
@@ -354,7 +364,9 @@ void enqueue_order(sdm_context* c, bool full, bool energy = false) {
     if (!full) {
         k_refresh<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->x64, c->orig_s, c->qs, c->g,
                                                 c->posq_s, c->arec, c->F, c->bb_lo, c->bb_hi,
-                                                c->par64, rec);
+                                                c->par64, rec,
+                                                lists_kept(c) ? c->xb : nullptr,
+                                                c->dmax);
     } else {
     k_bin<<<g2(N, 256, R), 256, 0, s>>>(N, Np, c->x64, c->key, c->ccount, c->cellrank, c->g,
                                         c->ncx, c->ncy, c->ncz, c->NC);
@@ -363,7 +375,7 @@ void enqueue_order(sdm_context* c, bool full, bool energy = false) {
     k_cellsort<<<g2(c->NC, 128, R), 128, 0, s>>>(N, Np, c->NC, c->cstart, c->orig_s, c->slot_of,
                                                 c->x64, c->qs, c->sh, c->se, c->posq_s,
                                                 c->par_s, c->arec, c->F, c->g, c->par64, rec);
-    k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi);
+    k_bbox<<<dim3(c->NB, R), 64, 0, s>>>(N, Np, c->NB, c->posq_s, c->bb_lo, c->bb_hi, c->dmax);
     }
 }
 
@@ -371,12 +383,27 @@ void enqueue_order(sdm_context* c, bool full, bool energy = false) {
 // work (fewer than 8 workgroups per CU)
 bool few_blocks(const sdm_context* c) { return (long)c->NB * c->R < 2048; }
 
+// build radius and reuse threshold of the tile lists for a skin of tuning.list_skin nm
+TileRadius tile_radius(const sdm_context* c) {
+    const float skin = lists_kept(c) ? c->tune.list_skin : 0.f;
+    const float rl = std::sqrt(c->list_r2) + skin;
+    TileRadius tr;
+    tr.r2 = rl * rl;
+    tr.oix = c->g.oix - skin;
+    tr.oiy = c->g.oiy - skin;
+    tr.oiz = c->g.oiz - skin;
+    tr.reuse_d2 = skin > 0.f ? 0.25f * skin * skin : -1.f;
+    return tr;
+}
+
 // tile lists of the block pair kernels and of the FP64 energy kernel (the grouped
 // kernel finds its j-blocks itself: with it, lists are built for energy evaluations only,
-// as half lists)
+// as half lists).  With a skin, k_tiles keeps a replica's lists while its atoms have
+// moved less than half the skin since they were built.
 void enqueue_tiles(sdm_context* c, bool energy) {
     if (c->tune.pair_group != 0 && !energy) return;
     const int half = c->tune.pair_group != 0 ? 1 : c->tune.pair_newton3;
+    const TileRadius tr = tile_radius(c);
     // One workgroup per (replica, i-block).  Its time is the serial path through it, so
     // when the workgroups do not fill the GPU (fewer than 8 per CU) each gets 8 waves for
     // the candidate masks, otherwise 4 (measured: one replica of 22,750 atoms +6%, 22
@@ -384,7 +411,7 @@ void enqueue_tiles(sdm_context* c, bool energy) {
 #define SDM_TILES(NW)                                                                         \
     k_tiles<NW><<<dim3(c->NB, c->R), 64 * NW, 0, c->stream>>>(                                \
         c->N, c->Np, c->NB, c->maxj, c->posq_s, c->bb_lo, c->bb_hi, c->tiles, c->tmask,       \
-        c->ntiles, c->list_r2, c->g, half, c->flags)
+        c->ntiles, c->dmax, c->xb, tr, c->g, half, c->flags)
     if (few_blocks(c)) SDM_TILES(8);
     else SDM_TILES(4);
 #undef SDM_TILES
@@ -587,7 +614,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     k_scalar<<<R, 256, 0, s>>>(c->NB * E64_SPLIT, c->nbw, c->LB, c->e64, c->e_bond, c->u_part, c->state,
                                c->rec, c->flags, advance ? 1 : 0, energy ? 1 : 0,
                                c->hist_every, c->hist_cap, c->uhist, c->e_rest,
-                               c->ncen + c->nali, c->e_t1, c->e_t2, c->ntb);
+                               c->ncen + c->nali, c->e_t1, c->e_t2, c->ntb, c->dmax);
     if (c->L > 0 && !components)
         k_mix<<<dim3(c->maxc + c->LB, R), 64, 0, s>>>(N, Np, c->L, c->maxc, c->lig, c->cls,
                                                        c->orig_s, c->cand, c->ncand, c->dB,
@@ -682,7 +709,7 @@ int tile_capacity(const sdm_system_desc* sys, int NB) {
     const double pi = 3.14159265358979323846;
     double V = sys->box[0] * sys->box[1] * sys->box[2];
     double vb = 64.0 * V / sys->n_atoms;
-    double reach = sys->cutoff + 1e-3 + 2.0 * 1.5 * std::cbrt(3.0 * vb / (4.0 * pi));
+    double reach = sys->cutoff + 1e-3 + SDM_MAX_LIST_SKIN + 2.0 * 1.5 * std::cbrt(3.0 * vb / (4.0 * pi));
     double nb = NB * (4.0 / 3.0) * pi * reach * reach * reach / V;
     return std::max(1, std::min(NB, std::max(64, (int)std::ceil(3.0 * nb))));
 }
@@ -899,6 +926,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     A(orig_s, RNp);
     A(slot_of, RNp); A(key, RNp); A(bb_lo, (size_t)R * NB); A(bb_hi, (size_t)R * NB);
     A(tiles, (size_t)R * NB * c->maxj); A(tmask, (size_t)R * NB * c->maxj);
+    A(xb, RNp); A(dmax, (size_t)R * DISP_STRIDE);
     A(ntiles, (size_t)R * NB); A(F, 3 * RNp);
     A(e_pair, (size_t)R * NB * 32); A(e_bond, (size_t)R * c->nbw); A(u_part, (size_t)R * c->LB);
     A(e64, (size_t)R * NB * E64_SPLIT); A(rec64, RNp);
@@ -1699,6 +1727,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
     if (t->side_fork < 0 || t->side_fork > 3)
         return fail(SDM_ERR_INVALID, "side_fork must be 0, 1, 2 or 3");
+    if (!(t->list_skin >= 0.f && t->list_skin <= (float)SDM_MAX_LIST_SKIN))
+        return fail(SDM_ERR_INVALID, "list_skin must be in [0, 0.1] nm");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -1803,6 +1833,26 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
             best_ms = total[k];
             best = finalists[k];
         }
+    // then the Verlet skin of the tile lists for the chosen configuration (block kernels
+    // only): a wider skin keeps the lists longer but the pair kernel evaluates more pairs
+    // outside the cutoff.  Two rounds again, ranked by the sum.
+    if (rc == SDM_OK && best.pair_group == 0) {
+        const float skins[] = {0.f, 0.01f, 0.02f, 0.03f, 0.04f};
+        constexpr int NS = sizeof(skins) / sizeof(skins[0]);
+        double tsk[NS] = {};
+        for (int round = 0; round < 2 && rc == SDM_OK; round++)
+            for (int k = 0; k < NS && rc == SDM_OK; k++) {
+                sdm_tuning t = best;
+                t.list_skin = skins[k];
+                double ms = 0.0;
+                rc = trial(t, 3 * steps, ms);
+                tsk[k] += ms;
+            }
+        int kb = 0;
+        for (int k = 1; k < NS; k++)
+            if (tsk[k] < tsk[kb]) kb = k;
+        if (rc == SDM_OK) best.list_skin = skins[kb];
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->hist_every = hist_every;

@@ -47,6 +47,22 @@ struct Geom {
     double go[3], gs[3];
 };
 
+// Tile lists kept over several steps (a Verlet skin).  The lists are built with the
+// radius r_l = r_c + skin + 1e-3 nm (r2), and with oix..oiy as Geom's but for that radius
+// (a shift exact for every pair within r_l).  A pair within r_c now was within
+// r_c + d_i + d_j when the lists were built (d: each atom's displacement since, as a
+// minimum image), so the lists hold every pair within r_c while 2 max d <= skin:
+// reuse_d2 = (skin / 2)^2 (negative: build on every call).
+struct TileRadius {
+    float r2, oix, oiy, oiz, reuse_d2;
+};
+// a block displacement record that forces the next build (after a sort, or non-finite)
+constexpr float DISP_STALE = 3.0e38f;
+// floats between consecutive replicas' displacement records (dmax[r * DISP_STRIDE]): one
+// 256-byte line each, so that the blocks' atomics of different replicas do not queue on
+// one line (all on one line cost 62 us per launch at 22 x 356 blocks)
+constexpr int DISP_STRIDE = 64;
+
 // Slot-ordered FP64 record of the energy kernel: the position in the same periodic image
 // as the slot's FP32 record (so one tile shift serves both), and the parameters.
 struct Rec64 {
@@ -172,10 +188,13 @@ __global__ void k_cellsort(int N, int Np, int NC, const int* __restrict__ cstart
     }
 }
 
-// bounding box of each 64-slot block (one wave per block).
+// bounding box of each 64-slot block (one wave per block).  After a sort the slot
+// order is new, so the replica's displacement record (dmax, read by k_tiles) is set to a
+// value that makes the next tile-list build unconditional.
 __global__ __launch_bounds__(64) void k_bbox(int N, int Np, int NB,
                                              const float4* __restrict__ posq_s,
-                                             float4* __restrict__ bb_lo, float4* __restrict__ bb_hi) {
+                                             float4* __restrict__ bb_lo, float4* __restrict__ bb_hi,
+                                             float* __restrict__ dmax) {
     int r = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
     int s = b * 64 + lane;
     bool v = s < N;
@@ -187,6 +206,7 @@ __global__ __launch_bounds__(64) void k_bbox(int N, int Np, int NB,
     if (lane == 0) {
         bb_lo[(size_t)r * NB + b] = make_float4(lx, ly, lz, 0.f);
         bb_hi[(size_t)r * NB + b] = make_float4(hx, hy, hz, 0.f);
+        if (b == 0) dmax[(size_t)r * DISP_STRIDE] = DISP_STALE;
     }
 }
 
@@ -206,7 +226,9 @@ __global__ __launch_bounds__(64) void k_refresh(int N, int Np, int NB,
                                                 float4* __restrict__ bb_lo,
                                                 float4* __restrict__ bb_hi,
                                                 const double4* __restrict__ par64,
-                                                Rec64* __restrict__ rec) {
+                                                Rec64* __restrict__ rec,
+                                                const float4* __restrict__ xb,
+                                                float* __restrict__ dmax) {
     int r = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
     int sl = b * 64 + lane;
     bool v = sl < N;  // slot b * 64 (lane 0) is always below N
@@ -243,6 +265,31 @@ __global__ __launch_bounds__(64) void k_refresh(int N, int Np, int NB,
         bb_lo[(size_t)r * NB + b] = make_float4(lx, ly, lz, 0.f);
         bb_hi[(size_t)r * NB + b] = make_float4(hx, hy, hz, 0.f);
     }
+    // kept tile lists (xb: the slot coordinates when they were built): the block's
+    // largest squared displacement since, as a minimum image (the block may have moved
+    // to another image of the box; non-finite counts as stale), into the replica's
+    // maximum by a device-scope atomic max (non-negative floats order as their bits).
+    // k_scalar sets it back to 0 at the end of each evaluation; where no k_scalar runs in
+    // between (pair timing, pair counts) it can only be larger, so lists are rebuilt
+    // sooner, never later.  (A last-block reduction needs a device-scope fence per block,
+    // which writes back the XCD's L2: +0.2 ms per step at 22 x 22,750 atoms.)  One atomic
+    // per block: 356 per replica at 22,750 atoms.
+    if (xb) {
+        float d2 = 0.f;
+        if (v) {
+            const float4 q = xb[o];
+            float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            dx -= g.bx * rintf(dx * g.ibx);
+            dy -= g.by * rintf(dy * g.iby);
+            dz -= g.bz * rintf(dz * g.ibz);
+            d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 <= 1.0e30f)) d2 = DISP_STALE;
+        }
+        d2 = wave_max(d2);
+        if (lane == 0)
+            atomicMax(reinterpret_cast<unsigned*>(dmax) + (size_t)r * DISP_STRIDE,
+                      __float_as_uint(d2));
+    }
 }
 
 __device__ __forceinline__ float bbox_gap2(float4 lo_i, float4 hi_i, float4 lo_j, float4 hi_j,
@@ -276,14 +323,18 @@ __device__ __forceinline__ float max8(float v) {
 }
 
 // tile list of one i-block: every j-block (j >= i for the half list) whose box is within
-// rc (+slack) of the i-block's box, ascending, each with a 64-bit mask of the j atoms
-// that lie within rc of the box of at least one of the block's eight 8-slot groups
-// (atoms outside all of them cannot have an in-cutoff partner in the i-block; the
-// union of the small boxes is a much tighter region than the block's box).  Half list:
-// in the diagonal tile an atom only counts for groups that start below it.
+// the list radius r_l (r_c + skin + slack, TileRadius) of the i-block's box, ascending,
+// each with a 64-bit mask of the j atoms that lie within r_l of the box of at least one
+// of the block's eight 8-slot groups (atoms outside all of them cannot have a partner
+// within r_l in the i-block; the union of the small boxes is a much tighter region than
+// the block's box).  Half list: in the diagonal tile an atom only counts for groups that
+// start below it.
 // NW waves per i-block: the candidate j-blocks (box test) are listed in LDS, each
 // wave computes the atom masks of every NW-th candidate, and wave 0 compacts the
 // non-empty ones in candidate order.  Results do not depend on NW.
+// Kept lists (tr.reuse_d2 >= 0): the workgroup returns at once while no atom of the
+// replica has moved more than half the skin since the build (dmax); a build stores the
+// block's slot coordinates in xb.
 constexpr int TILE_CAND = 256;  // candidates per round
 template <int NW>  // waves per i-block workgroup
 __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int maxj,
@@ -292,7 +343,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
                                                const float4* __restrict__ bb_hi,
                                                int* __restrict__ tiles,
                                                unsigned long long* __restrict__ tmask,
-                                               int* __restrict__ ntiles, float list_r2, Geom g,
+                                               int* __restrict__ ntiles,
+                                               const float* __restrict__ dmax,
+                                               float4* __restrict__ xb, TileRadius tr, Geom g,
                                                int half, int* __restrict__ flags) {
     __shared__ float4 sub[8][2];
     __shared__ int cand[TILE_CAND];
@@ -300,6 +353,10 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
     __shared__ int ncand_s, base_s;
     int r = blockIdx.y, ib = blockIdx.x;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float list_r2 = tr.r2;
+    // kept lists: every workgroup of the replica reads the same record (written by the
+    // slot refresh or the sort before this launch), so all of them keep or all rebuild
+    if (tr.reuse_d2 >= 0.f && dmax[(size_t)r * DISP_STRIDE] <= tr.reuse_d2) return;
     const float4* lo = bb_lo + (size_t)r * NB;
     const float4* hi = bb_hi + (size_t)r * NB;
     const float4* P = posq_s + (size_t)r * Np;
@@ -309,6 +366,7 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
         int is = ib * 64 + lane;
         bool v = is < N;
         float4 p = P[min(is, N - 1)];
+        if (tr.reuse_d2 >= 0.f && v) xb[(size_t)r * Np + is] = p;  // the build coordinates
         float lx = min8(v ? p.x : 3.0e38f), ly = min8(v ? p.y : 3.0e38f), lz = min8(v ? p.z : 3.0e38f);
         float hx = max8(v ? p.x : -3.0e38f), hy = max8(v ? p.y : -3.0e38f), hz = max8(v ? p.z : -3.0e38f);
         if ((lane & 7) == 0) {
@@ -323,8 +381,7 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
         int s = jb2 * 64 + lane;
         float4 p = P[s];
         // one periodic shift for the whole tile when it is exact for every pair within
-        // the list radius (Geom::oix, as in the pair kernel); otherwise the minimum image
-        // per group
+        // the list radius (TileRadius::oix); otherwise the minimum image per group
         float4 lj = lo[jb2], hj = hi[jb2];
         float dcx = 0.5f * (li.x + hi_i.x - lj.x - hj.x);
         float dcy = 0.5f * (li.y + hi_i.y - lj.y - hj.y);
@@ -332,9 +389,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiles(int N, int Np, int NB, int ma
         float sx = g.bx * rintf(dcx * g.ibx), sy = g.by * rintf(dcy * g.iby),
               sz = g.bz * rintf(dcz * g.ibz);
         bool one_image =
-            fabsf(dcx - sx) + 0.5f * (hi_i.x - li.x + hj.x - lj.x) < g.oix &&
-            fabsf(dcy - sy) + 0.5f * (hi_i.y - li.y + hj.y - lj.y) < g.oiy &&
-            fabsf(dcz - sz) + 0.5f * (hi_i.z - li.z + hj.z - lj.z) < g.oiz;
+            fabsf(dcx - sx) + 0.5f * (hi_i.x - li.x + hj.x - lj.x) < tr.oix &&
+            fabsf(dcy - sy) + 0.5f * (hi_i.y - li.y + hj.y - lj.y) < tr.oiy &&
+            fabsf(dcz - sz) + 0.5f * (hi_i.z - li.z + hj.z - lj.z) < tr.oiz;
         if (one_image) {
             p.x += sx;
             p.y += sy;
@@ -2767,9 +2824,13 @@ __global__ __launch_bounds__(256) void k_scalar(
     DevState* __restrict__ state, DevRecord* __restrict__ rec, int* __restrict__ flags,
     int advance, int energy, int hist_every, int hist_cap, double* __restrict__ uhist,
     const double* __restrict__ e_rest, int nrest, const double* __restrict__ e_t1,
-    const double* __restrict__ e_t2, int ntb) {
+    const double* __restrict__ e_t2, int ntb, float* __restrict__ dmax) {
     __shared__ double red[256];
     int r = blockIdx.x, t = threadIdx.x;
+    // the displacement record of kept tile lists restarts with the next slot refresh:
+    // this evaluation's k_tiles has read it and kept or rebuilt the lists (after a sort,
+    // rebuilt), and the next refresh measures the displacement since that build again
+    if (t == 0) dmax[(size_t)r * DISP_STRIDE] = 0.f;
     double s0 = 0.0, s1 = 0.0;
     // energy == 0: the pair kernel left the pair energies out; U0, U1 and pot_energy
     // keep the values of the last evaluation that had them (nothing else depends on U0)

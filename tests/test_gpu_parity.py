@@ -437,7 +437,8 @@ def test_forces_only_steps_match_energy_steps_on_ladder(hip_device, use_graph, s
 @pytest.mark.parametrize("field,value", [("pair_newton3", 2), ("use_graph", 2),
                                          ("pair_group", 4), ("pair_waves_per_group", 3),
                                          ("sort_interval", 0), ("energy_every_step", -1),
-                                         ("pair_split", 3), ("side_fork", 4)])
+                                         ("pair_split", 3), ("side_fork", 4),
+                                         ("list_skin", 0.2), ("list_skin", -0.01)])
 def test_tuning_out_of_range_refused(hip_device, field, value):
     """sdm_set_tuning refuses values outside each field's documented set and keeps the
     previous tuning."""
@@ -473,3 +474,68 @@ def test_side_work_placement_gives_same_bits(hip_device, use_graph):
     for x, pe in xs[1:]:
         assert np.array_equal(x, xs[0][0]) and pe == xs[0][1]
 
+
+
+def _skin_run(frames, params, tuning, skin, n1=16, n2=5):
+    """n1 steps, positions set again (replica r takes frame r + 1), n2 more steps."""
+    R = len(frames)
+    ctx = _native.NativeContext(frames[0], R)
+    ctx.set_tuning(**tuning, list_skin=skin)
+    for r in range(R):
+        ctx.set_params(r, params[r])
+        ctx.set_positions(r, frames[r].positions)
+        ctx.set_velocities(r, frames[r].velocities)
+    ctx.step(n1)
+    x1 = np.stack([ctx.get_positions(r) for r in range(R)])
+    for r in range(R):
+        ctx.set_positions(r, frames[(r + 1) % R].positions)
+    ctx.step(n2)
+    es = ctx.energies()
+    assert all(e["flags"] == 0 for e in es)
+    assert ctx.get_tuning()["list_skin"] == pytest.approx(skin)
+    return x1, np.stack([ctx.get_positions(r) for r in range(R)]), [e["pot_energy"] for e in es]
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+@pytest.mark.parametrize("kernel", ["block_n3", "block_full", "grp16"])
+def test_list_skin_gives_same_bits(hip_device, kernel, use_graph):
+    """Tile lists kept over steps with a Verlet skin (built for r_c + skin, kept while no
+    atom of the replica has moved half the skin): the pair kernel still applies the exact
+    cutoff, the pairs added by the skin contribute exact zeros, and with one wave per
+    i-block the i-force sums see the same nonzero terms in the same order, so the
+    trajectory is the same bits as with lists built on every step (the FP64 pair energy
+    agrees to rounding: its kernel's four waves per i-block take other shares of the
+    longer tile lists).  Skins of 0.02 nm
+    (lists kept about one step at 300 K and 1 fs) and 0.1 nm (several steps), full sorts
+    every 7 steps, positions set again in the middle (the lists are rebuilt from the new
+    coordinates).  The grouped kernel ignores the skin (its lists are built every time)."""
+    frames = [systems.make_config1(frame=k) for k in range(3)]
+    params = systems.params_config3(3)
+    tuning = dict(KERNELS[kernel], sort_interval=7, use_graph=use_graph)
+    runs = []
+    for skin in (0.0, 0.02, 0.1):
+        print(f"case kernel={kernel} use_graph={use_graph} list_skin={skin}", flush=True)
+        runs.append(_skin_run(frames, params, tuning, skin))
+    for x1, x2, pe in runs[1:]:
+        assert np.array_equal(x1, runs[0][0])
+        assert np.array_equal(x2, runs[0][1])
+        for a, b in zip(pe, runs[0][2]):
+            assert abs(a - b) <= 1e-13 * max(1.0, abs(b)), (a, b)
+
+
+def test_list_skin_with_split_waves(hip_device):
+    """With several waves per i-block the skin's extra tiles change which wave takes
+    which tile, so the i forces are rounded to fixed point in other groups: the
+    trajectory agrees with lists built every step to within that rounding."""
+    frames = [systems.make_config1(frame=k) for k in range(3)]
+    params = systems.params_config3(3)
+    tuning = dict(KERNELS["block_n3_split4"], sort_interval=7, use_graph=0)
+    print("case block_n3_split4 list_skin 0 / 0.05", flush=True)
+    a = _skin_run(frames, params, tuning, 0.0, n1=8, n2=4)
+    b = _skin_run(frames, params, tuning, 0.05, n1=8, n2=4)
+    for xa, xb in ((a[0], b[0]), (a[1], b[1])):
+        d = float(np.max(np.abs(xa - xb)))
+        print(f"max |dx| {d:.2e} nm", flush=True)
+        assert d < 1e-7
+    for pa, pb in zip(a[2], b[2]):
+        assert abs(pa - pb) <= 1e-7 * max(1.0, abs(pa))
