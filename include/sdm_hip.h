@@ -309,6 +309,28 @@ typedef struct {
    call in the reference, whose DMS reader adds these forces to the System. */
 int sdm_set_forcefield_terms(sdm_context* ctx, const sdm_forcefield_terms* terms);
 
+/* Virtual sites: massless atoms placed from two or three others.  The reference's
+   integrator places them after every step (ReferenceStochasticDynamicsSDM.cpp:264,
+   OpenCLSDMKernels.cpp:376-377, through OpenMM 7's VirtualSite classes), and its DMS reader
+   makes them from the virtual_lc2 / virtual_lc3 / virtual_out3 tables
+   (desmonddmsfile75.py _addVirtualSitesToSystem).  Site k is atom atoms[4k] (mass 0),
+   placed from atoms[4k+1..4k+3] (the fourth is ignored for SDM_VSITE_AVERAGE2) with
+   weights w = weights[3k..3k+2]:
+     SDM_VSITE_AVERAGE2      x = w0 x1 + w1 x2                (TwoParticleAverageSite)
+     SDM_VSITE_AVERAGE3      x = w0 x1 + w1 x2 + w2 x3        (ThreeParticleAverageSite)
+     SDM_VSITE_OUT_OF_PLANE  x = x1 + w0 r12 + w1 r13 + w2 (r12 x r13), r1k = xk - x1
+                                                              (OutOfPlaneSite)
+   Each step moves the forces on the sites to their parents (the transpose of the site's
+   Jacobian) before the Langevin update and places the sites after it; sdm_get_forces
+   reports forces moved the same way (zero on the sites).  Parents must have mass and not
+   be sites; a site and its parents share one displacement class.  n = 0 removes them.
+   No counterpart call in the reference, where the System holds the sites. */
+#define SDM_VSITE_AVERAGE2 0
+#define SDM_VSITE_AVERAGE3 1
+#define SDM_VSITE_OUT_OF_PLANE 2
+int sdm_set_virtual_sites(sdm_context* ctx, int32_t n, const int32_t* kind,
+                          const int32_t* atoms, const double* weights);
+
 /* Relative tolerance of the position constraints (|r|^2 within tol of d^2).  Replaces
    Integrator::setConstraintTolerance, whose value the reference hands to the constraint
    solver every step (ReferenceStochasticDynamicsSDM.cpp:252, getConstraintTolerance()). */

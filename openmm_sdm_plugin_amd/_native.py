@@ -122,7 +122,7 @@ EXPORTS = [
     "sdm_get_u_history", "sdm_count_pairs", "sdm_time_pair_kernel", "sdm_autotune",
     "sdm_get_tuning", "sdm_set_tuning", "sdm_du_dparams_frames", "sdm_set_restraints",
     "sdm_set_forcefield_terms", "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance",
-    "sdm_du_dparams_frames_noalloc",
+    "sdm_du_dparams_frames_noalloc", "sdm_set_virtual_sites",
     "registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories",
 ]
 
@@ -172,6 +172,7 @@ def load(path: str = LIB_PATH):
     lib.sdm_set_forcefield_terms.argtypes = [vp, C.POINTER(sdm_forcefield_terms)]
     lib.sdm_set_cm_motion_removal.argtypes = [vp, C.c_int32]
     lib.sdm_set_constraint_tolerance.argtypes = [vp, C.c_double]
+    lib.sdm_set_virtual_sites.argtypes = [vp, C.c_int32, _ip, _ip, _dp]
     for f in ("sdm_create", "sdm_destroy", "sdm_num_replicas", "sdm_set_params", "sdm_get_params",
               "sdm_set_time", "sdm_set_nonbonded_params", "sdm_set_positions",
               "sdm_get_positions", "sdm_set_velocities", "sdm_get_velocities",
@@ -181,7 +182,7 @@ def load(path: str = LIB_PATH):
               "sdm_time_pair_kernel", "sdm_autotune", "sdm_get_tuning", "sdm_set_tuning",
               "sdm_du_dparams_frames", "sdm_set_restraints", "sdm_set_forcefield_terms",
               "sdm_set_cm_motion_removal", "sdm_set_constraint_tolerance",
-              "sdm_du_dparams_frames_noalloc"):
+              "sdm_du_dparams_frames_noalloc", "sdm_set_virtual_sites"):
         getattr(lib, f).restype = C.c_int
     for f in ("registerPlatforms", "registerKernelFactories", "registerSDMHIPKernelFactories"):
         getattr(lib, f).restype = None
@@ -270,6 +271,8 @@ class NativeContext:
             self.set_forcefield_terms(s.terms)
             if getattr(s.terms, "remove_cm_motion", False):
                 self.set_cm_motion_removal(1)
+        if getattr(s, "virtual_sites", None) is not None:
+            self.set_virtual_sites(s.virtual_sites)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -342,6 +345,18 @@ class NativeContext:
 
     def set_constraint_tolerance(self, tol):
         check(self._lib.sdm_set_constraint_tolerance(self.h, float(tol)))
+
+    def set_virtual_sites(self, vs):
+        """Virtual sites (an object with kind (n,), atoms (n,4): site, p1, p2, p3 and
+        weights (n,3), as sdm_set_virtual_sites takes them); None removes them."""
+        if vs is None or len(vs.kind) == 0:
+            check(self._lib.sdm_set_virtual_sites(self.h, 0, None, None, None))
+            return
+        kind = np.ascontiguousarray(vs.kind, dtype=np.int32)
+        atoms = np.ascontiguousarray(np.asarray(vs.atoms).reshape(-1, 4), dtype=np.int32)
+        w = np.ascontiguousarray(np.asarray(vs.weights).reshape(-1, 3), dtype=np.float64)
+        check(self._lib.sdm_set_virtual_sites(self.h, len(kind), kind.ctypes.data_as(_I),
+                                              atoms.ctypes.data_as(_I), w.ctypes.data_as(_D)))
 
     def set_forcefield_terms(self, ff, constraint_tol=0.0):
         """Torsions, impropers, positional restraints, 1-4 exceptions and constraints of a

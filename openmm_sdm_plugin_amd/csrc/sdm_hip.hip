@@ -156,6 +156,13 @@ struct sdm_context {
     double* uhist = nullptr;
     int hist_every = 0, hist_cap = 1;
     long long* Fb_sep = nullptr;
+    // virtual sites (sdm_set_virtual_sites): kind, (site, p1, p2, p3), 3 weights each
+    int nvs = 0;
+    int* vs_kind = nullptr;
+    int4* vs_at = nullptr;
+    double* vs_w = nullptr;
+    std::vector<int> h_vs_kind, h_vs_at;
+    std::vector<double> h_vs_w;
     double* dFd = nullptr;
     unsigned long long* counters = nullptr;
     double* ke_out = nullptr;
@@ -627,6 +634,10 @@ void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullp
     if (c->cm_every > 0)
         k_cm_remove<<<c->R, 1024, 0, c->stream>>>(c->N, c->Np, c->cm_every, c->mass, c->v64,
                                                    c->state);
+    if (c->nvs > 0)  // forces on virtual sites to their parents
+        k_vsite_forces<<<g2(c->nvs, 256, c->R), 256, 0, c->stream>>>(
+            c->nvs, c->Np, c->vs_kind, c->vs_at, c->vs_w, c->x64, c->slot_of, c->F, c->Fb,
+            c->flags);
     k_langevin<<<g2(c->N, 256, c->R), 256, 0, c->stream>>>(c->N, c->Np, c->invm, c->F, c->Fb,
                                                            c->x64, c->v64, c->rec, c->state,
                                                            c->slot_of, c->cflag, c->dxs);
@@ -634,6 +645,9 @@ void enqueue_step(sdm_context* c, bool full, bool energy, hipEvent_t pe0 = nullp
         k_constrain<<<g2(c->ncl, 256, c->R), 256, 0, c->stream>>>(
             c->Np, c->ncl, c->cl_start, c->cl_atoms, c->con_start, c->con_ij, c->con_d2,
             c->con_tol, c->invm, c->dxs, c->x64, c->v64, c->rec, c->flags);
+    if (c->nvs > 0)  // the sites placed from their parents' new positions
+        k_vsite_pos<<<g2(c->nvs, 256, c->R), 256, 0, c->stream>>>(c->nvs, c->Np, c->vs_kind,
+                                                                  c->vs_at, c->vs_w, c->x64);
 }
 
 int check_launch() {
@@ -1528,6 +1542,120 @@ int sdm_compute_components(sdm_context* c) {
     return SDM_OK;
 }
 
+}  // extern "C"
+
+// The distribution of k_vsite_forces in FP64 on the host, for forces read back (x: one
+// replica's positions, [3][Np]; out: [N][3]).  A site's force is left at zero, so forces a
+// step has already distributed on the device pass through unchanged.
+static void vsite_distribute_host(const sdm_context* c, const std::vector<double>& x,
+                                  double* out) {
+    const size_t Np = (size_t)c->Np;
+    for (int k = 0; k < c->nvs; k++) {
+        const int* a = &c->h_vs_at[4 * (size_t)k];
+        const double* w = &c->h_vs_w[3 * (size_t)k];
+        double f[3], g1[3], g2[3], g3[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; d++) {
+            f[d] = out[3 * (size_t)a[0] + d];
+            out[3 * (size_t)a[0] + d] = 0.0;
+        }
+        if (c->h_vs_kind[k] == VSITE_OUT_OF_PLANE) {
+            double r12[3], r13[3];
+            for (int d = 0; d < 3; d++) {
+                r12[d] = x[d * Np + a[2]] - x[d * Np + a[1]];
+                r13[d] = x[d * Np + a[3]] - x[d * Np + a[1]];
+            }
+            const double c2[3] = {r13[1] * f[2] - r13[2] * f[1], r13[2] * f[0] - r13[0] * f[2],
+                                  r13[0] * f[1] - r13[1] * f[0]};
+            const double c3[3] = {f[1] * r12[2] - f[2] * r12[1], f[2] * r12[0] - f[0] * r12[2],
+                                  f[0] * r12[1] - f[1] * r12[0]};
+            for (int d = 0; d < 3; d++) {
+                g2[d] = w[0] * f[d] + w[2] * c2[d];
+                g3[d] = w[1] * f[d] + w[2] * c3[d];
+                g1[d] = f[d] - g2[d] - g3[d];
+            }
+        } else {
+            for (int d = 0; d < 3; d++) {
+                g1[d] = w[0] * f[d];
+                g2[d] = w[1] * f[d];
+                if (c->h_vs_kind[k] == VSITE_AVERAGE3) g3[d] = w[2] * f[d];
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            out[3 * (size_t)a[1] + d] += g1[d];
+            out[3 * (size_t)a[2] + d] += g2[d];
+            if (c->h_vs_kind[k] != VSITE_AVERAGE2) out[3 * (size_t)a[3] + d] += g3[d];
+        }
+    }
+}
+
+extern "C" {
+
+int sdm_set_virtual_sites(sdm_context* c, int32_t n, const int32_t* kind, const int32_t* atoms,
+                          const double* weights) {
+    if (!c) return fail(SDM_ERR_INVALID, "null context");
+    if (n < 0) return fail(SDM_ERR_INVALID, "negative virtual-site count");
+    if (n > 0 && (!kind || !atoms || !weights))
+        return fail(SDM_ERR_INVALID, "null virtual-site array");
+    const int N = c->N;
+    std::vector<double> mass(N);
+    std::vector<int> cls(N);
+    CHK(download(c, mass.data(), c->mass, N));
+    CHK(download(c, cls.data(), c->cls, N));
+    std::vector<char> site(N, 0);
+    std::vector<int> at(4 * (size_t)n);
+    for (int k = 0; k < n; k++) {
+        if (kind[k] != SDM_VSITE_AVERAGE2 && kind[k] != SDM_VSITE_AVERAGE3 &&
+            kind[k] != SDM_VSITE_OUT_OF_PLANE)
+            return fail(SDM_ERR_INVALID, "unknown virtual-site kind");
+        const int np = kind[k] == SDM_VSITE_AVERAGE2 ? 2 : 3;
+        for (int j = 0; j <= np; j++)
+            if (atoms[4 * k + j] < 0 || atoms[4 * k + j] >= N)
+                return fail(SDM_ERR_INVALID, "virtual-site atom index out of range");
+        for (int j = 0; j < 3; j++)
+            if (!std::isfinite(weights[3 * k + j]))
+                return fail(SDM_ERR_INVALID, "virtual-site weights must be finite");
+        const int a = atoms[4 * k];
+        if (site[a]) return fail(SDM_ERR_INVALID, "an atom is defined as a virtual site twice");
+        if (mass[a] != 0.0) return fail(SDM_ERR_INVALID, "a virtual site must have mass 0");
+        site[a] = 1;
+        for (int j = 0; j < 4; j++) at[4 * k + j] = atoms[4 * k + (j <= np ? j : 1)];
+    }
+    for (int k = 0; k < n; k++) {
+        const int np = kind[k] == SDM_VSITE_AVERAGE2 ? 2 : 3;
+        for (int j = 1; j <= np; j++) {
+            const int p = at[4 * k + j];
+            if (site[p]) return fail(SDM_ERR_INVALID, "a virtual site's parent is a virtual site");
+            if (!(mass[p] > 0.0)) return fail(SDM_ERR_INVALID, "a virtual site's parent has no mass");
+            if (cls[p] != cls[at[4 * k]])
+                return fail(SDM_ERR_INVALID,
+                            "a virtual site and its parents must share one displacement");
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, c->vs_kind);
+    dfree(c, c->vs_at);
+    dfree(c, c->vs_w);
+    c->vs_kind = nullptr;
+    c->vs_at = nullptr;
+    c->vs_w = nullptr;
+    c->nvs = 0;
+    c->h_vs_kind.assign(kind, kind + n);
+    c->h_vs_at = at;
+    c->h_vs_w.assign(weights, weights + 3 * (size_t)n);
+    if (n > 0) {
+        CHK(dalloc(c, &c->vs_kind, n));
+        CHK(dalloc(c, &c->vs_at, n));
+        CHK(dalloc(c, &c->vs_w, 3 * (size_t)n));
+        CHK(upload(c, c->vs_kind, kind, n));
+        CHK(upload(c, c->vs_at, reinterpret_cast<const int4*>(at.data()), n));
+        CHK(upload(c, c->vs_w, weights, 3 * (size_t)n));
+    }
+    c->nvs = n;
+    c->components_valid = false;
+    drop_graph(c);  // the step's kernels change
+    return SDM_OK;
+}
+
 int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
     CHK(check_replica(c, r, false));
     if (!out) return fail(SDM_ERR_INVALID, "null output");
@@ -1540,12 +1668,20 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
     CHK(download(c, f.data(), c->F + off, f.size()));
     CHK(download(c, slot.data(), c->slot_of + (size_t)r * Np, Np));
     if (c->Fb) CHK(download(c, fbond.data(), c->Fb + off, fbond.size()));
+    // forces on virtual sites are reported moved to their parents, as the step uses them
+    auto sites = [&]() -> int {
+        if (c->nvs == 0) return SDM_OK;
+        std::vector<double> x(3 * (size_t)Np);
+        CHK(download(c, x.data(), c->x64 + off, x.size()));
+        vsite_distribute_host(c, x, out);
+        return SDM_OK;
+    };
     if (which == SDM_FORCE_HYBRID && !c->components_valid) {
         for (int i = 0; i < N; i++)
             for (int d = 0; d < 3; d++)
                 out[3 * i + d] = (double)f[(size_t)d * Np + slot[i]] / SDM_FIX_SCALE +
                                  fbond[(size_t)d * Np + i];
-        return SDM_OK;
+        return sites();
     }
     if (!c->components_valid)
         return fail(SDM_ERR_STATE, "force components require sdm_compute_components");
@@ -1569,7 +1705,7 @@ int sdm_get_forces(sdm_context* c, int32_t r, int32_t which, double* out) {
             }
             out[3 * i + d] = v;
         }
-    return SDM_OK;
+    return sites();
 }
 
 int sdm_kinetic_energy(sdm_context* c, double* out) {

@@ -3049,6 +3049,137 @@ __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* _
     }
 }
 
+// Virtual sites (massless atoms placed from others; the reference's integrator places
+// them after every step, ReferenceStochasticDynamicsSDM.cpp:264, from the definitions the
+// DMS reader makes, desmonddmsfile75.py _addVirtualSitesToSystem).  Site k: at[k] = (site,
+// p1, p2, p3), weights w[3k..3k+2]:
+//   VSITE_AVERAGE2:      x = w0 x1 + w1 x2
+//   VSITE_AVERAGE3:      x = w0 x1 + w1 x2 + w2 x3
+//   VSITE_OUT_OF_PLANE:  x = x1 + w0 r12 + w1 r13 + w2 (r12 x r13),  r1k = xk - x1
+// (coordinates as stored, without periodic images, as OpenMM 7 places them).
+constexpr int VSITE_AVERAGE2 = 0, VSITE_AVERAGE3 = 1, VSITE_OUT_OF_PLANE = 2;
+
+__device__ __forceinline__ void vsite_cross(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// before the Langevin update: each site's force (fixed-point accumulator plus the FP64
+// bonded forces) moves to its parents by the transpose of the site's Jacobian, through
+// fixed-point atomics (parents are never sites, so no site's own entry is written by
+// another thread); the site's entries are left at zero.  Out of plane: a parent k's share
+// is w f plus w2 times the cross-product term's derivative (r13 x f for p2, f x r12 for p3),
+// and p1 takes the rest, so the three shares sum to f.
+__global__ __launch_bounds__(256) void k_vsite_forces(int nvs, int Np,
+                                                      const int* __restrict__ kind,
+                                                      const int4* __restrict__ at,
+                                                      const double* __restrict__ w,
+                                                      const double* __restrict__ x64,
+                                                      const int* __restrict__ slot_of,
+                                                      long long* __restrict__ F,
+                                                      double* __restrict__ Fb,
+                                                      int* __restrict__ flags) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (k >= nvs) return;
+    const int4 a = at[k];
+    const int K = kind[k];
+    const double w0 = w[3 * k], w1 = w[3 * k + 1], w2 = w[3 * k + 2];
+    long long* Fr = F + (size_t)r * 3 * Np;
+    double* Fbr = Fb ? Fb + (size_t)r * 3 * Np : nullptr;
+    const int* S = slot_of + (size_t)r * Np;
+    const double* X = x64 + (size_t)r * 3 * Np;
+    const int ss = S[a.x];
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        f[d] = from_fixed(Fr[d * Np + ss]);
+        Fr[d * Np + ss] = 0;
+        if (Fbr) {
+            f[d] += Fbr[d * Np + a.x];
+            Fbr[d * Np + a.x] = 0.0;
+        }
+    }
+    double g1[3], g2[3], g3[3];
+    if (K == VSITE_OUT_OF_PLANE) {
+        double r12[3], r13[3], c2[3], c3[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            r12[d] = X[d * Np + a.z] - X[d * Np + a.y];
+            r13[d] = X[d * Np + a.w] - X[d * Np + a.y];
+        }
+        vsite_cross(r13, f, c2);
+        vsite_cross(f, r12, c3);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            g2[d] = w0 * f[d] + w2 * c2[d];
+            g3[d] = w1 * f[d] + w2 * c3[d];
+            g1[d] = f[d] - g2[d] - g3[d];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            g1[d] = w0 * f[d];
+            g2[d] = w1 * f[d];
+            g3[d] = K == VSITE_AVERAGE3 ? w2 * f[d] : 0.0;
+        }
+    }
+    int bad = 0;
+    const int s1 = S[a.y], s2 = S[a.z];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Fr[d * Np + s1]),
+                  (unsigned long long)to_fixed(g1[d], &bad));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Fr[d * Np + s2]),
+                  (unsigned long long)to_fixed(g2[d], &bad));
+    }
+    if (K != VSITE_AVERAGE2) {
+        const int s3 = S[a.w];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&Fr[d * Np + s3]),
+                      (unsigned long long)to_fixed(g3[d], &bad));
+    }
+    if (bad) atomicOr(&flags[r], 2);
+}
+
+// after the update and the constraints: the sites' positions from their parents'
+__global__ __launch_bounds__(256) void k_vsite_pos(int nvs, int Np, const int* __restrict__ kind,
+                                                   const int4* __restrict__ at,
+                                                   const double* __restrict__ w,
+                                                   double* __restrict__ x64) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (k >= nvs) return;
+    const int4 a = at[k];
+    const int K = kind[k];
+    const double w0 = w[3 * k], w1 = w[3 * k + 1], w2 = w[3 * k + 2];
+    double* X = x64 + (size_t)r * 3 * Np;
+    double p1[3], p2[3], p3[3] = {0.0, 0.0, 0.0}, o[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        p1[d] = X[d * Np + a.y];
+        p2[d] = X[d * Np + a.z];
+        if (K != VSITE_AVERAGE2) p3[d] = X[d * Np + a.w];
+    }
+    if (K == VSITE_OUT_OF_PLANE) {
+        double r12[3], r13[3], c[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            r12[d] = p2[d] - p1[d];
+            r13[d] = p3[d] - p1[d];
+        }
+        vsite_cross(r12, r13, c);
+#pragma unroll
+        for (int d = 0; d < 3; d++) o[d] = p1[d] + w0 * r12[d] + w1 * r13[d] + w2 * c[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            o[d] = w0 * p1[d] + w1 * p2[d] + (K == VSITE_AVERAGE3 ? w2 * p3[d] : 0.0);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) X[d * Np + a.x] = o[d];
+}
+
 // centre-of-mass motion removal (one workgroup per replica) on steps whose index (the
 // step count before the step, as OpenMM's CMMotionRemover tests it in the
 // updateContextState call that starts the step, LangevinIntegratorSDM.cpp:158) is a

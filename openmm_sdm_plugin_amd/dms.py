@@ -22,9 +22,16 @@ internal units (nm, ps, amu, e, kJ/mol, rad):
   nonbonded    charge, sigma, epsilon per particle; arithmetic (Lorentz-Berthelot) or
                geometric (OPLS) sigma rule from nonbonded_info (`:1042-1077`)
 
+  virtual sites virtual_lc2: x = (1-c1) x1 + c1 x2; virtual_lc3: x = (1-c1-c2) x1 + c1 x2 +
+               c2 x3; virtual_out3: x = x1 + c1 r12 + c2 r13 + c3 (r12 x r13), c3 passed on
+               unconverted as the reference does (`:852-915`); the engine places them and
+               moves their forces (sdm_set_virtual_sites)
+
 Multi-file inputs are concatenated with index offsets as in the reference (`:79-117`).
-Unsupported terms raise NotImplementedError as the reference does (CMAP, virtual sites,
-flat-bottom stretch/angle/improper terms, non-12-6 van der Waals, Drude particles).
+Unsupported terms raise NotImplementedError as the reference does (CMAP, fdat3 virtual
+sites, flat-bottom stretch/angle/improper terms, non-12-6 van der Waals, Drude particles);
+so do virtual-site tables other than lc2, lc3, out3 and fdat3, which the reference's
+reader passes over without placing the sites.
 
 `to_sdm_system(ff, ...)` turns a DMSForceField into the engine's SDMSystem with the
 reaction field in place of Ewald/PME (the engine's only long-range treatment).
@@ -35,6 +42,7 @@ import dataclasses
 import math
 import os
 import sqlite3
+import types
 from typing import List, Optional
 
 import numpy as np
@@ -43,6 +51,8 @@ KCAL = 4.184  # kJ per kcal
 ANG = 0.1  # nm per angstrom
 DEG = math.pi / 180.0
 
+# the virtual-site tables the reference's reader places sites from (:852-911)
+VSITE_TABLES = ("virtual_lc2_term", "virtual_lc3_term", "virtual_out3_term")
 NONBONDED_METHODS = ("NoCutoff", "CutoffNonPeriodic", "CutoffPeriodic", "Ewald", "PME", "LJPME")
 
 
@@ -118,6 +128,13 @@ class DMSForceField:
     remove_cm_motion: bool
     bonded_force_group: int = 1
     nonbonded_force_group: int = 2
+    # virtual sites: kind (0 two-particle average, 1 three-particle average, 2 out of
+    # plane), atoms (v, 4): site, p1, p2, p3; weights (v, 3)
+    vsite_kind: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros(0, dtype=np.int32))
+    vsite_atoms: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 4), dtype=np.int32))
+    vsite_weights: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3)))
 
     @property
     def n(self):
@@ -254,10 +271,14 @@ class DesmondDMSFile:
             if "cmap_term" in tables and \
                     conn.execute("SELECT COUNT(*) FROM cmap_term").fetchone()[0] != 0:
                 raise NotImplementedError("CMAP terms are not supported")
+            if "virtual_fdat3_term" in tables:
+                raise NotImplementedError("OpenMM does not currently support fdat3-style "
+                                          "virtual sites")
             if any(t.startswith("virtual_") and t.endswith("_term") and
                    conn.execute("SELECT COUNT(*) FROM %s" % t).fetchone()[0] != 0
-                   for t in tables if t != "virtual_term"):
-                raise NotImplementedError("Virtual sites are not supported")
+                   for t in tables if t not in VSITE_TABLES and t != "virtual_term"):
+                raise NotImplementedError("Virtual sites other than lc2, lc3 and out3 are "
+                                          "not supported")
         return opls
 
     def _rows(self, q, table):
@@ -407,6 +428,27 @@ class DesmondDMSFile:
                     mass[b] = hydrogenMass
                     mass[a] -= transfer
 
+        # virtual sites, table by table as the reference adds them (:852-915)
+        vk, va, vw = [], [], []
+        q = ("SELECT p0, p1, p2, c1 FROM virtual_lc2_term INNER JOIN virtual_lc2_param "
+             "ON virtual_lc2_term.param=virtual_lc2_param.id")
+        for off, (p0, p1, p2, c1) in self._rows(q, "virtual_lc2_term"):
+            vk.append(0)
+            va.append((p0 + off, p1 + off, p2 + off, p1 + off))
+            vw.append((1.0 - c1, c1, 0.0))
+        q = ("SELECT p0, p1, p2, p3, c1, c2 FROM virtual_lc3_term INNER JOIN "
+             "virtual_lc3_param ON virtual_lc3_term.param=virtual_lc3_param.id")
+        for off, (p0, p1, p2, p3, c1, c2) in self._rows(q, "virtual_lc3_term"):
+            vk.append(1)
+            va.append((p0 + off, p1 + off, p2 + off, p3 + off))
+            vw.append((1.0 - c1 - c2, c1, c2))
+        q = ("SELECT p0, p1, p2, p3, c1, c2, c3 FROM virtual_out3_term INNER JOIN "
+             "virtual_out3_param ON virtual_out3_term.param=virtual_out3_param.id")
+        for off, (p0, p1, p2, p3, c1, c2, c3) in self._rows(q, "virtual_out3_term"):
+            vk.append(2)
+            va.append((p0 + off, p1 + off, p2 + off, p3 + off))
+            vw.append((c1, c2, c3))
+
         def arr(v, shape, dtype=np.float64):
             return np.asarray(v, dtype=dtype).reshape(shape)
         return DMSForceField(
@@ -426,7 +468,8 @@ class DesmondDMSFile:
             posre_h_atoms=arr(ph_a, (-1,), np.int32), posre_h_x0=arr(ph_x, (-1, 3)),
             posre_h_hk=arr(ph_k, (-1, 3)), box=None if box is None else box.copy(),
             nonbonded_method=nonbondedMethod, cutoff=float(nonbondedCutoff),
-            remove_cm_motion=bool(removeCMMotion))
+            remove_cm_motion=bool(removeCMMotion), vsite_kind=arr(vk, (-1,), np.int32),
+            vsite_atoms=arr(va, (-1, 4), np.int32), vsite_weights=arr(vw, (-1, 3)))
 
 
 def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=None,
@@ -461,6 +504,10 @@ def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=No
         np.asarray(displacement, dtype=np.float64).reshape(n, 3)
     if ligand is None:
         ligand = np.nonzero(np.any(disp != 0.0, axis=1))[0]
+    vsites = None
+    if len(ff.vsite_kind):
+        vsites = types.SimpleNamespace(kind=ff.vsite_kind.copy(), atoms=ff.vsite_atoms.copy(),
+                                       weights=ff.vsite_weights.copy())
     return SDMSystem(
         box=ff.box.copy() if periodic else np.zeros(3),
         cutoff=float(ff.cutoff if cutoff is None else cutoff),
@@ -469,4 +516,4 @@ def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=No
         bonds=ff.bonds.copy(), bond_r0=ff.bond_r0.copy(), bond_k=ff.bond_k.copy(),
         angles=ff.angles.copy(), angle_t0=ff.angle_t0.copy(), angle_k=ff.angle_k.copy(),
         displacement=disp, positions=x, velocities=v, periodic=periodic,
-        ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff)
+        ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff, virtual_sites=vsites)

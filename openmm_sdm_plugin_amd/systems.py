@@ -109,6 +109,8 @@ class SDMSystem:
     restraints: list = dataclasses.field(default_factory=list)
     # further force-field terms and constraints (a dms.DMSForceField), or None
     terms: Optional[object] = None
+    # virtual sites (kind (v,), atoms (v,4): site, parents; weights (v,3)), or None
+    virtual_sites: Optional[object] = None
 
     @property
     def n(self) -> int:

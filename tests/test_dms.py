@@ -11,7 +11,7 @@ import sqlite3
 import numpy as np
 import pytest
 
-from openmm_sdm_plugin_amd.dms import ANG, KCAL, DesmondDMSFile
+from openmm_sdm_plugin_amd.dms import ANG, KCAL, DesmondDMSFile, to_sdm_system
 
 REF_EXAMPLES = "/root/reference/example"
 
@@ -131,6 +131,58 @@ def test_synthetic_dms_terms(tmp_path):
     assert ff.epsilon[0] == pytest.approx(0.152 * 4.184)
     assert ff.exclusions.shape == (9, 2)
     assert ff.nonbonded_method == "PME" and ff.cutoff == 0.9
+
+
+VSITE_SQL = [
+    # a charge site on the water bisector (lc3), a site between C1 and C2 (lc2), and an
+    # out-of-plane site of C2-C3-C4 (out3); each a massless particle appended to the file
+    "INSERT INTO particle VALUES (7, 'M', 0, 'WAT', 1, 'A', 0.1, 0.1, 0.0, 0, 0, 0, 0.0, "
+    "-0.1, 1)",
+    "INSERT INTO particle VALUES (8, 'X', 0, 'LIG', 2, 'B', 3.7, 0, 0, 0, 0, 0, 0.0, 0.0, 1)",
+    "INSERT INTO particle VALUES (9, 'Y', 0, 'LIG', 2, 'B', 5.0, 0, 1, 0, 0, 0, 0.0, 0.0, 1)",
+    "CREATE TABLE virtual_lc3_param (id INTEGER PRIMARY KEY, c1 FLOAT, c2 FLOAT)",
+    "INSERT INTO virtual_lc3_param VALUES (0, 0.12, 0.13)",
+    "CREATE TABLE virtual_lc3_term (p0 INTEGER, p1 INTEGER, p2 INTEGER, p3 INTEGER, "
+    "param INTEGER)",
+    "INSERT INTO virtual_lc3_term VALUES (7, 0, 1, 2, 0)",
+    "CREATE TABLE virtual_lc2_param (id INTEGER PRIMARY KEY, c1 FLOAT)",
+    "INSERT INTO virtual_lc2_param VALUES (0, 0.4)",
+    "CREATE TABLE virtual_lc2_term (p0 INTEGER, p1 INTEGER, p2 INTEGER, param INTEGER)",
+    "INSERT INTO virtual_lc2_term VALUES (8, 3, 4, 0)",
+    "CREATE TABLE virtual_out3_param (id INTEGER PRIMARY KEY, c1 FLOAT, c2 FLOAT, c3 FLOAT)",
+    "INSERT INTO virtual_out3_param VALUES (0, 0.3, 0.2, 0.5)",
+    "CREATE TABLE virtual_out3_term (p0 INTEGER, p1 INTEGER, p2 INTEGER, p3 INTEGER, "
+    "param INTEGER)",
+    "INSERT INTO virtual_out3_term VALUES (9, 4, 5, 6, 0)",
+]
+
+
+def test_virtual_sites(tmp_path):
+    """The reference's reader makes TwoParticleAverageSite(p1, p2, 1-c1, c1),
+    ThreeParticleAverageSite(p1, p2, p3, 1-c1-c2, c1, c2) and OutOfPlaneSite(p1, p2, p3,
+    c1, c2, c3) (desmonddmsfile75.py:877, :892, :907), lc2 tables first."""
+    path = _make_dms(str(tmp_path / "v.dms"), extra=VSITE_SQL)
+    ff = DesmondDMSFile(path).createSystem(nonbondedMethod="CutoffPeriodic")
+    assert ff.n == 10 and ff.mass[7] == 0.0
+    assert ff.vsite_kind.tolist() == [0, 1, 2]
+    assert ff.vsite_atoms[0, :3].tolist() == [8, 3, 4]
+    assert ff.vsite_atoms[1].tolist() == [7, 0, 1, 2]
+    assert ff.vsite_atoms[2].tolist() == [9, 4, 5, 6]
+    assert np.allclose(ff.vsite_weights[0, :2], [0.6, 0.4])
+    assert np.allclose(ff.vsite_weights[1], [0.75, 0.12, 0.13])
+    assert np.allclose(ff.vsite_weights[2], [0.3, 0.2, 0.5])
+    # offsets of a second file
+    ff2 = DesmondDMSFile([path, path]).createSystem(nonbondedMethod="CutoffPeriodic")
+    assert ff2.vsite_atoms[ff2.vsite_kind == 1].tolist() == [[7, 0, 1, 2], [17, 10, 11, 12]]
+    s = to_sdm_system(ff, DesmondDMSFile(path).getPositions())
+    assert s.virtual_sites is not None and s.virtual_sites.kind.tolist() == [0, 1, 2]
+    with pytest.raises(NotImplementedError):
+        DesmondDMSFile(_make_dms(str(tmp_path / "f3.dms"), extra=[
+            "CREATE TABLE virtual_fdat3_term (p0 INTEGER)"])).createSystem()
+    with pytest.raises(NotImplementedError):
+        DesmondDMSFile(_make_dms(str(tmp_path / "o.dms"), extra=[
+            "CREATE TABLE virtual_lc4_term (p0 INTEGER)",
+            "INSERT INTO virtual_lc4_term VALUES (0)"])).createSystem()
 
 
 def test_multi_file_offsets(tmp_path):
