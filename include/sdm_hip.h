@@ -75,7 +75,17 @@ typedef struct {
                                    first example, example/test.py:63) */
     double cutoff;              /* nm; periodic: box >= 2*cutoff in every dimension */
     double eps_rf;              /* reaction-field dielectric */
+    int32_t lj_rule;            /* SDM_LJ_ARITHMETIC (0, NonbondedForce's Lorentz-Berthelot
+                                   sigma_ij = (s_i + s_j)/2) or SDM_LJ_GEOMETRIC (1,
+                                   sigma_ij = sqrt(s_i s_j): the CustomNonbondedForce the
+                                   reference's DMS reader builds for OPLS,
+                                   desmonddmsfile75.py:780-793); epsilon_ij =
+                                   sqrt(e_i e_j) either way.  Geometric runs on the block
+                                   Newton-3 pair kernel (sdm_tuning.pair_group 0,
+                                   pair_newton3 1). */
 } sdm_system_desc;
+#define SDM_LJ_ARITHMETIC 0
+#define SDM_LJ_GEOMETRIC 1
 
 /* Integrator parameters of one replica: the LangevinIntegratorSDM fields
    (LangevinIntegratorSDM.h:497-521, defaults LangevinIntegratorSDM.cpp:48-85). */

@@ -41,7 +41,7 @@ class sdm_system_desc(C.Structure):
         ("bond_atoms", _ip), ("bond_r0", _dp), ("bond_k", _dp), ("n_angles", C.c_int32),
         ("angle_atoms", _ip), ("angle_theta0", _dp), ("angle_k", _dp), ("displacement", _dp),
         ("box", C.c_double * 3), ("periodic", C.c_int32), ("cutoff", C.c_double),
-        ("eps_rf", C.c_double),
+        ("eps_rf", C.c_double), ("lj_rule", C.c_int32),
     ]
 
 
@@ -262,6 +262,7 @@ class NativeContext:
         desc.periodic = int(s.periodic)
         desc.cutoff = float(s.cutoff)
         desc.eps_rf = float(s.eps_rf)
+        desc.lj_rule = int(getattr(s, "lj_rule", 0))
         h = C.c_void_p()
         check(lib.sdm_create(C.byref(desc), n_replicas, C.byref(h)))
         self.h = h

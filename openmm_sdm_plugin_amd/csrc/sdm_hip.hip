@@ -488,12 +488,17 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
         c->tmask,                                                                           \
         c->ntiles, c->bb_lo, c->bb_hi, c->ex_start, c->ex_list, c->max_ex, c->F, c->e_pair, \
         c->flags, pc, c->g
-#define SDM_LAUNCH_N3(W)                                                                      \
+#define SDM_LAUNCH_N3_G(W, GEO)                                                               \
     do {                                                                                      \
         dim3 gr((waves + W - 1) / W);                                                         \
-        if (count) k_pair_n3<W, true, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);            \
-        else if (energy) k_pair_n3<W, false, true><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);     \
-        else k_pair_n3<W, false, false><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);                \
+        if (count) k_pair_n3<W, true, true, GEO><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);       \
+        else if (energy) k_pair_n3<W, false, true, GEO><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS); \
+        else k_pair_n3<W, false, false, GEO><<<gr, 64 * W, lds, s>>>(SDM_N3_ARGS);           \
+    } while (0)
+#define SDM_LAUNCH_N3(W)                                                                      \
+    do {                                                                                      \
+        if (c->g.geo) SDM_LAUNCH_N3_G(W, true);                                               \
+        else SDM_LAUNCH_N3_G(W, false);                                                       \
     } while (0)
         switch (c->tune.pair_waves_per_group) {
             case 2: SDM_LAUNCH_N3(2); break;
@@ -501,6 +506,7 @@ void enqueue_pair(sdm_context* c, bool count, bool energy = true) {
             default: SDM_LAUNCH_N3(1); break;
         }
 #undef SDM_LAUNCH_N3
+#undef SDM_LAUNCH_N3_G
 #undef SDM_N3_ARGS
         return;
     }
@@ -738,12 +744,13 @@ int upload_nb_params(sdm_context* c, const double* q, const double* sig, const d
         if (!(sig[i] >= 0) || !(eps[i] >= 0) || !std::isfinite(q[i]))
             return fail(SDM_ERR_INVALID, "nonbonded parameters must be finite, sigma/epsilon >= 0");
         qs[i] = (float)(q[i] * sk);
-        sh[i] = (float)(0.5 * sig[i]);
+        const double sh64 = c->g.geo ? sig[i] : 0.5 * sig[i];  // the entry each rule combines
+        sh[i] = (float)sh64;
         se[i] = (float)(2.0 * std::sqrt(eps[i]));
         q64[i] = q[i] * sk;
         s64[i] = sig[i];
         e64[i] = eps[i];
-        p64[i] = double4{q[i] * sk, 0.5 * sig[i], 2.0 * std::sqrt(eps[i]), 0.0};
+        p64[i] = double4{q[i] * sk, sh64, 2.0 * std::sqrt(eps[i]), 0.0};
     }
     CHK(upload(c, c->qs, qs.data(), Np));
     CHK(upload(c, c->sh, sh.data(), Np));
@@ -778,6 +785,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
             if (!(sys->box[d] >= 2.0 * sys->cutoff))
                 return fail(SDM_ERR_INVALID, "the cutoff distance cannot be greater than half the periodic box size");
     if (!(sys->eps_rf > 0)) return fail(SDM_ERR_INVALID, "eps_rf must be positive");
+    if (sys->lj_rule != SDM_LJ_ARITHMETIC && sys->lj_rule != SDM_LJ_GEOMETRIC)
+        return fail(SDM_ERR_INVALID, "lj_rule must be SDM_LJ_ARITHMETIC or SDM_LJ_GEOMETRIC");
     for (int i = 0; i < N; i++)
         if (!(sys->mass[i] >= 0)) return fail(SDM_ERR_INVALID, "masses must be >= 0");
     // exclusions -> CSR
@@ -883,6 +892,7 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
     c->maxj = tile_capacity(sys, c->NB);
     c->maxc = std::min(c->NB, 256 * std::max(1, c->ncls) + 64);
     Geom& g = c->g;
+    g.geo = sys->lj_rule == SDM_LJ_GEOMETRIC ? 1 : 0;
     c->periodic = sys->periodic ? 1 : 0;
     c->reg_box.assign(6 * (size_t)n_replicas, 0.0);
     for (int r = 0; r < n_replicas; r++) {  // no positions yet: empty boxes
@@ -1865,6 +1875,9 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "side_fork must be 0, 1, 2 or 3");
     if (!(t->list_skin >= 0.f && t->list_skin <= (float)SDM_MAX_LIST_SKIN))
         return fail(SDM_ERR_INVALID, "list_skin must be in [0, 0.1] nm");
+    if (c->g.geo && (t->pair_group != 0 || t->pair_newton3 != 1))
+        return fail(SDM_ERR_INVALID, "the geometric LJ rule runs on the block Newton-3 pair "
+                                     "kernel (pair_group 0, pair_newton3 1)");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (t->cell_cells_per_dim != c->tune.cell_cells_per_dim) CHK(setup_cells(c, t->cell_cells_per_dim));
     c->tune = *t;
@@ -1926,6 +1939,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
         for (int w : waves)
             for (const auto& kn : kernels) {
                 if (rc != SDM_OK) break;
+                if (c->g.geo && (kn[0] != 0 || kn[1] != 1)) continue;  // see sdm_set_tuning
                 sdm_tuning t = c->tune;
                 t.pair_waves_per_group = w;
                 t.pair_group = kn[0];

@@ -41,6 +41,10 @@ struct Geom {
     // and at least L/2 >= r_c as evaluated, so it is left out either way.  oix..oiz hold
     // L - r_l less a 1e-3 nm margin for FP32 rounding (1e30 if not periodic)
     float oix, oiy, oiz;
+    // LJ sigma combining: 0 Lorentz-Berthelot, 1 geometric (OPLS).  The per-atom FP32 and
+    // FP64-record sigma entries hold s/2 for the first, s for the second (the FP64 sigma
+    // array always s); the FP32 block kernel takes it as a template parameter
+    int geo;
     // sort grid: cell c_d = floor((w_d - go_d) * gs_d), clamped to the grid, w the
     // (wrapped, if periodic) coordinate.  Periodic: go = 0, gs = cells / L; otherwise
     // the grid spans a region around the positions (atoms outside take edge cells)
@@ -568,8 +572,9 @@ __device__ __forceinline__ float pair_one(float4 pi, float2 qi, float4 pj, float
 // mask outside the cutoff (a never-taken branch per pair would cost more and would keep
 // the scheduler from overlapping consecutive pairs).  Masked lanes may hold inf or NaN
 // before the mask (r = 0 for an unused slot); the AND yields +0.
-// ENERGY = false: forces only (the energy terms and their sums are left out)
-template <bool ENERGY = true>
+// ENERGY = false: forces only (the energy terms and their sums are left out).  GEO: the
+// geometric sigma rule, qi.x and qj.x holding sigma itself (Geom::geo)
+template <bool ENERGY = true, bool GEO = false>
 __device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, float2 qj,
                                              bool incl, bool minimg, const Geom& g, PairAcc& a,
                                              float& dx, float& dy, float& dz) {
@@ -585,8 +590,13 @@ __device__ __forceinline__ float pair_masked(float4 pi, float2 qi, float4 pj, fl
     unsigned m = (incl && r2 < g.rc2f) ? 0xFFFFFFFFu : 0u;
     float ir = __frsqrt_rn(r2);
     float ir2 = ir * ir;
-    float sig = qi.x + qj.x;
-    float s2 = sig * sig * ir2;
+    float s2;
+    if constexpr (GEO) {
+        s2 = (qi.x * qj.x) * ir2;
+    } else {
+        float sig = qi.x + qj.x;
+        s2 = sig * sig * ir2;
+    }
     float s6 = s2 * s2 * s2;
     float es6 = (qi.y * qj.y) * s6;
     float qq = pi.w * pj.w;
@@ -924,7 +934,7 @@ __global__ __launch_bounds__(64 * WPW) void k_pair(
 // At least 5 waves per SIMD (at most 96 VGPRs, allocated in granules of 8): the energy
 // variant otherwise takes 97, which leaves room for 4.  The forces-only variant takes
 // 84; held to 80 for 6 waves it spills 24 bytes per lane and runs 3.3% slower.
-template <int WPW, bool COUNT, bool ENERGY>
+template <int WPW, bool COUNT, bool ENERGY, bool GEO = false>
 __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_n3(
     int R, int N, int Np, int NB, int maxj, int split, const float4* __restrict__ arec,
     const int* __restrict__ orig_s, const int* __restrict__ slot_of, const int* __restrict__ tiles,
@@ -1078,7 +1088,8 @@ __global__ __launch_bounds__(64 * WPW) __attribute__((amdgpu_waves_per_eu(5))) v
                     bool incl = vi && k < nvalid;
                     if (EX) incl = incl && !((em >> (jj[k] & 63)) & 1ull);
                     float dx, dy, dz;
-                    float fr = pair_masked<ENERGY>(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
+                    float fr =
+                        pair_masked<ENERGY, GEO>(ps, qi, pj, qj, incl, MI, g, acc, dx, dy, dz);
                     v[3 * k] = -fr * dx;
                     v[3 * k + 1] = -fr * dy;
                     v[3 * k + 2] = -fr * dz;
@@ -1745,8 +1756,13 @@ __global__ __launch_bounds__(64) void k_energy64(
             if (r2 < g.rc2) {
                 double ir = rsqrt(r2);
                 double ir2 = ir * ir;
-                double sig = I.sh + J.sh;
-                double s2 = sig * sig * ir2;
+                double s2;
+                if (g.geo) {  // Geom::geo: the records hold sigma itself
+                    s2 = I.sh * J.sh * ir2;
+                } else {
+                    double sig = I.sh + J.sh;
+                    s2 = sig * sig * ir2;
+                }
                 double s6 = s2 * s2 * s2;
                 double qq = I.qk * J.qk;
                 e += (I.es * J.es) * s6 * (s6 - 1.0) + qq * (ir + g.krf * r2 - g.crf);
@@ -2486,7 +2502,7 @@ __device__ __forceinline__ Atom64 load_atom64(const double* __restrict__ X, int 
 __device__ __forceinline__ double delta_pair(const Atom64& I, const Atom64& J, const Geom& g,
                                              double& fx, double& fy, double& fz, int* inside_y) {
     double qq = I.qk * J.qk;
-    double sig = 0.5 * (I.sig + J.sig);
+    double sig = g.geo ? sqrt(I.sig * J.sig) : 0.5 * (I.sig + J.sig);
     double eps4 = 4.0 * I.se * J.se;
     double ex, fr_x, ey, fr_y;
     double dx = min_image(I.x - J.x, g.Lx, g.iLx);
@@ -3422,8 +3438,12 @@ __global__ __launch_bounds__(64) void k_du_dparams(
         if (cj == cs || excluded(ex_start, ex_list, s, j)) continue;
         double xj, yj, zj;
         load_frame_atom(X, j, xj, yj, zj);
-        double sig = 0.5 * (sig64[s] + sig64[j]);
+        double sig = g.geo ? sqrt(sig64[s] * sig64[j]) : 0.5 * (sig64[s] + sig64[j]);
         double eps = sqrt(eps64[s] * eps64[j]);
+        // d sigma_sj / d sigma_s: 1/2, or sigma_j / (2 sigma_sj) for the geometric rule
+        // (zero with sigma_sj = 0, where the LJ terms vanish)
+        const double dsds = g.geo ? (sig > 0.0 ? sig64[j] / (2.0 * sig) : 0.0) : 0.5;
+        const bool lj = !g.geo || sig > 0.0;
         for (int geo = 0; geo < 2; geo++) {  // geo 0: at x (sign -), geo 1: at x+d (sign +)
             double dx = min_image((xs + (geo ? cdispl[3 * cs] : 0.0)) - (xj + (geo ? cdispl[3 * cj] : 0.0)), g.Lx, g.iLx);
             double dy = min_image((ys + (geo ? cdispl[3 * cs + 1] : 0.0)) - (yj + (geo ? cdispl[3 * cj + 1] : 0.0)), g.Ly, g.iLy);
@@ -3437,7 +3457,7 @@ __global__ __launch_bounds__(64) void k_du_dparams(
             double s12 = s6 * s6;
             // q64 holds q*sqrt(k_e): d(k_e q_s q_j f)/dq_s = sqrt(k_e) * q64[j] * f
             dq += sgn * ske * q64[j] * (ir + g.krf * r2 - g.crf);
-            dsig += sgn * 0.5 * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
+            if (lj) dsig += sgn * dsds * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
             if (eps > 0.0) deps += sgn * 4.0 * (s12 - s6) * eps64[j] / (2.0 * eps);
         }
     }

@@ -12,15 +12,20 @@ internal units (nm, ps, amu, e, kJ/mol, rad):
   constraints  the above, constraint_ah* (`:615-625`) and the H-H constraint of
                constraint_hoh (`:627-637`)
   torsions     dihedral_trig: one periodic term k (1 + cos(n phi - phi0)) per nonzero
-               fc_n, n = 0..6 (`:666-682`)
+               fc_n, n = 0..6 (`:666-682`); with OPLS the reference's CustomTorsionForce
+               f cos(n phi - phi0) (`:652-656`, `:679-680`), written as the two periodic
+               terms (f/2, phi0) and (-f/2, phi0 + pi), whose sum it is exactly
   impropers    improper_harm: k (phi - phi0)^2 (`:684-717`)
   exclusions   exclusion (`:803-808`)
   exceptions   pair_12_6_es: qq = q_ij, sigma = (a/b)^(1/6), epsilon = b^2/(4a)
                (`:818-845`); each must also be an exclusion
   posre        posre_harmflatbottom: hk (dist - tol)^2 beyond tol, hk = fc/2 (`:972-1017`);
                posre_harm: sum_d hk_d (x_d - x0_d)^2, hk = fc/2 (`:917-970`)
-  nonbonded    charge, sigma, epsilon per particle; arithmetic (Lorentz-Berthelot) or
-               geometric (OPLS) sigma rule from nonbonded_info (`:1042-1077`)
+  nonbonded    charge, sigma, epsilon per particle; Lorentz-Berthelot, or with
+               createSystem(OPLS=True) the geometric sigma of the reference's
+               CustomNonbondedForce (`:780-801`: LJ sqrt(s1 s2), sqrt(e1 e2), plain cutoff,
+               no long-range correction; charges stay in the NonbondedForce), which the
+               engine evaluates in its pair kernels (sdm_system_desc.lj_rule)
 
   virtual sites virtual_lc2: x = (1-c1) x1 + c1 x2; virtual_lc3: x = (1-c1-c2) x1 + c1 x2 +
                c2 x3; virtual_out3: x = x1 + c1 r12 + c2 r13 + c3 (r12 x r13), c3 passed on
@@ -366,10 +371,16 @@ class DesmondDMSFile:
             for order, fc in enumerate(row[5:12]):
                 if fc == 0:
                     continue
-                tors.append(p)
-                tn.append(order)
-                tph.append(row[4] * DEG)
-                tk.append(fc * KCAL)
+                if opls:  # f cos(x) = f/2 (1 + cos x) - f/2 (1 + cos(x - pi))
+                    tors += [p, p]
+                    tn += [order, order]
+                    tph += [row[4] * DEG, row[4] * DEG + math.pi]
+                    tk += [0.5 * fc * KCAL, -0.5 * fc * KCAL]
+                else:
+                    tors.append(p)
+                    tn.append(order)
+                    tph.append(row[4] * DEG)
+                    tk.append(fc * KCAL)
         imps, it0, ik = [], [], []
         q = ("SELECT p0, p1, p2, p3, phi0, fc FROM improper_harm_term INNER JOIN "
              "improper_harm_param ON improper_harm_term.param=improper_harm_param.id")
@@ -493,9 +504,6 @@ def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=No
                                   "CutoffNonPeriodic)")
     if eps_rf is None:
         eps_rf = 78.5 if periodic else 78.3
-    if ff.opls:
-        raise NotImplementedError("geometric (OPLS) combining rules are not supported: the "
-                                  "pair kernels combine sigma arithmetically")
     n = ff.n
     x = np.asarray(positions, dtype=np.float64).reshape(n, 3)
     v = np.zeros((n, 3)) if velocities is None else \
@@ -516,4 +524,5 @@ def to_sdm_system(ff: DMSForceField, positions, velocities=None, displacement=No
         bonds=ff.bonds.copy(), bond_r0=ff.bond_r0.copy(), bond_k=ff.bond_k.copy(),
         angles=ff.angles.copy(), angle_t0=ff.angle_t0.copy(), angle_k=ff.angle_k.copy(),
         displacement=disp, positions=x, velocities=v, periodic=periodic,
-        ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff, virtual_sites=vsites)
+        ligand=np.asarray(ligand, dtype=np.int32), name=name, terms=ff, virtual_sites=vsites,
+        lj_rule=1 if ff.opls else 0)

@@ -111,6 +111,8 @@ class SDMSystem:
     terms: Optional[object] = None
     # virtual sites (kind (v,), atoms (v,4): site, parents; weights (v,3)), or None
     virtual_sites: Optional[object] = None
+    # LJ sigma combining: 0 Lorentz-Berthelot, 1 geometric (OPLS)
+    lj_rule: int = 0
 
     @property
     def n(self) -> int:

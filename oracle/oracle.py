@@ -41,7 +41,7 @@ class orc_scalars(C.Structure):
 class orc_nb(C.Structure):
     _fields_ = [("n", C.c_int), ("q", _dp), ("sigma", _dp), ("eps", _dp), ("n_excl", C.c_int),
                 ("excl", _ip), ("box", C.c_double * 3), ("periodic", C.c_int),
-                ("cutoff", C.c_double), ("eps_rf", C.c_double)]
+                ("cutoff", C.c_double), ("eps_rf", C.c_double), ("geometric", C.c_int)]
 
 
 class orc_bonded_t(C.Structure):
@@ -137,6 +137,7 @@ class Topology:
         nb.periodic = int(s.periodic if periodic is None else periodic)
         nb.cutoff = float(s.cutoff)
         nb.eps_rf = float(s.eps_rf)
+        nb.geometric = int(getattr(s, "lj_rule", 0))
         self.nb = nb
         ba, bap = _i(np.asarray(s.bonds).reshape(-1))
         br, brp = _d(s.bond_r0)

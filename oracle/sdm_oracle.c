@@ -208,6 +208,13 @@ static int excl_has(const excl_csr* e, int i, int j) {
 
 static inline double min_image(double d, double L) { return d - L * floor(d / L + 0.5); }
 
+/* the pair's LJ sigma: Lorentz-Berthelot, or geometric (OPLS: the reference's DMS reader
+   then moves LJ to CustomNonbondedForce "sigma12=sqrt(sigma1*sigma2)",
+   desmonddmsfile75.py:780-793, with epsilon 0 left in NonbondedForce, :798-801) */
+static inline double pair_sigma(const orc_nb* nb, int i, int j) {
+    return nb->geometric ? sqrt(nb->sigma[i] * nb->sigma[j]) : 0.5 * (nb->sigma[i] + nb->sigma[j]);
+}
+
 /* One pair (i, j): adds forces, returns energy; 0 and no force when r >= rc. */
 static inline double pair_ij(const orc_nb* nb, const double* xi, const double* xj, int i, int j,
                              double* Fi, double* Fj, int* inside) {
@@ -221,7 +228,7 @@ static inline double pair_ij(const orc_nb* nb, const double* xi, const double* x
     if (r2 >= nb->cutoff * nb->cutoff) return 0.0;
     *inside = 1;
     double fr;
-    double e = orc_pair_energy(sqrt(r2), nb->q[i] * nb->q[j], 0.5 * (nb->sigma[i] + nb->sigma[j]),
+    double e = orc_pair_energy(sqrt(r2), nb->q[i] * nb->q[j], pair_sigma(nb, i, j),
                                sqrt(nb->eps[i] * nb->eps[j]), nb->cutoff, nb->eps_rf, &fr);
     for (int k = 0; k < 3; k++) {
         if (Fi) Fi[k] += fr * d[k];
@@ -468,8 +475,11 @@ void orc_du_dparams(const orc_nb* nb, const double* x, const double* displ, int 
             if (is_displaced(displ, j) && j < i) continue;
             if (excl_has(&ex, i, j)) continue;
             if (pos[i] < 0 && pos[j] < 0) continue;
-            double sig = 0.5 * (nb->sigma[i] + nb->sigma[j]);
+            double sig = pair_sigma(nb, i, j);
             double eps = sqrt(nb->eps[i] * nb->eps[j]);
+            /* d sigma_ij / d sigma_i: 1/2, or sigma_j / (2 sigma_ij) for the geometric rule */
+            double dsi = nb->geometric ? (sig > 0 ? nb->sigma[j] / (2.0 * sig) : 0.0) : 0.5;
+            double dsj = nb->geometric ? (sig > 0 ? nb->sigma[i] / (2.0 * sig) : 0.0) : 0.5;
             /* per-geometry partials: g = +1 at y, -1 at x */
             double dEdqq = 0, dEdsig = 0, dEdeps = 0;
             for (int geo = 0; geo < 2; geo++) {
@@ -487,19 +497,19 @@ void orc_du_dparams(const orc_nb* nb, const double* x, const double* displ, int 
                 double s6 = pow(sr, 6), s12 = s6 * s6;
                 double sgn = geo ? 1.0 : -1.0;
                 dEdqq += sgn * ORC_ONE_4PI_EPS0 * (ir + krf * r2 - crf);
-                dEdsig += sgn * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
+                if (sig > 0) dEdsig += sgn * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
                 dEdeps += sgn * 4.0 * (s12 - s6);
             }
             if (pos[i] >= 0) {
                 int s = pos[i];
                 dq[s] += dEdqq * nb->q[j];
-                dsig[s] += 0.5 * dEdsig;
+                dsig[s] += dsi * dEdsig;
                 if (eps > 0) deps[s] += dEdeps * nb->eps[j] / (2.0 * eps);
             }
             if (pos[j] >= 0) {
                 int s = pos[j];
                 dq[s] += dEdqq * nb->q[i];
-                dsig[s] += 0.5 * dEdsig;
+                dsig[s] += dsj * dEdsig;
                 if (eps > 0) deps[s] += dEdeps * nb->eps[i] / (2.0 * eps);
             }
         }

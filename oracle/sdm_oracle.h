@@ -96,6 +96,9 @@ typedef struct {
     int periodic;
     double cutoff;
     double eps_rf;
+    int geometric; /* 0: sigma_ij = (s_i + s_j)/2 (NonbondedForce); 1: sqrt(s_i s_j), the
+                      CustomNonbondedForce the reference's DMS reader builds for OPLS
+                      (desmonddmsfile75.py:780-793) */
 } orc_nb;
 
 typedef struct {

@@ -6,6 +6,9 @@ GPU tests (the GPU machines do not have the reference):
   tests/golden/oa_g6_g3.npz     example/oa-g6-g3-align-restr_0_displaced.dms (230 atoms,
                                 vacuum), read with CutoffNonPeriodic and a 15 nm cutoff as
                                 example/test.py:63 does
+  tests/golden/oa_g6_g3_opls.npz the same file read with createSystem(OPLS=True): its
+                                nonbonded_info declares the geometric rule, which the
+                                reference's reader applies only when asked this way
 
 Run from the repository root:
 
@@ -22,13 +25,16 @@ sys.path.insert(0, ROOT)
 
 from openmm_sdm_plugin_amd.dms import DesmondDMSFile  # noqa: E402
 
-CASES = (("temoa-g1-g4.dms", "temoa_g1_g4.npz", "PME", 1.0),
-         ("oa-g6-g3-align-restr_0_displaced.dms", "oa_g6_g3.npz", "CutoffNonPeriodic", 15.0))
+CASES = (("temoa-g1-g4.dms", "temoa_g1_g4.npz", "PME", 1.0, False),
+         ("oa-g6-g3-align-restr_0_displaced.dms", "oa_g6_g3.npz", "CutoffNonPeriodic", 15.0,
+          False),
+         ("oa-g6-g3-align-restr_0_displaced.dms", "oa_g6_g3_opls.npz", "CutoffNonPeriodic",
+          15.0, True))
 
 
-def write(path, dest_name, method, cutoff):
+def write(path, dest_name, method, cutoff, opls):
     d = DesmondDMSFile(path)
-    ff = d.createSystem(nonbondedMethod=method, nonbondedCutoff=cutoff)
+    ff = d.createSystem(nonbondedMethod=method, nonbondedCutoff=cutoff, OPLS=opls)
     out = {k: v for k, v in dataclasses.asdict(ff).items() if isinstance(v, np.ndarray)}
     out["positions"] = d.getPositions()
     out["velocities"] = d.getVelocities()
@@ -40,10 +46,12 @@ def write(path, dest_name, method, cutoff):
     d.close()
 
 
-def main(folder):
-    for name, dest, method, cutoff in CASES:
-        write(os.path.join(folder, name), dest, method, cutoff)
+def main(folder, only=None):
+    for name, dest, method, cutoff, opls in CASES:
+        if only is None or dest in only:
+            write(os.path.join(folder, name), dest, method, cutoff, opls)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/example")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/example",
+         sys.argv[2:] or None)

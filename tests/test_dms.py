@@ -4,6 +4,7 @@ A synthetic DMS file is written with sqlite3 (the reference ships no reader test
 expected values follow desmonddmsfile75.py's conversions written out by hand.  The
 reference's example files, when present, are checked against aggregates computed
 straight from their tables with SQL (an independent path from the reader's)."""
+import dataclasses
 import math
 import os
 import sqlite3
@@ -183,6 +184,32 @@ def test_virtual_sites(tmp_path):
         DesmondDMSFile(_make_dms(str(tmp_path / "o.dms"), extra=[
             "CREATE TABLE virtual_lc4_term (p0 INTEGER)",
             "INSERT INTO virtual_lc4_term VALUES (0)"])).createSystem()
+
+
+def test_opls_torsions_and_rule(tmp_path):
+    """With OPLS=True the reference writes torsions as f cos(n phi - phi0) (a
+    CustomTorsionForce, desmonddmsfile75.py:652-656, :679-680): here two periodic terms per
+    fc, whose energy is that sum exactly; the engine gets the geometric sigma rule."""
+    from oracle import terms as OT
+    path = _make_dms(str(tmp_path / "o.dms"), vdw_rule="geometric")
+    d = DesmondDMSFile(path)
+    ff = d.createSystem(nonbondedMethod="CutoffPeriodic", OPLS=True)
+    assert ff.opls and ff.torsion_n.tolist() == [1, 1, 3, 3]
+    x = d.getPositions()
+    only_torsions = dataclasses.replace(
+        ff, impropers=np.zeros((0, 4), np.int32), improper_t0=np.zeros(0),
+        improper_k=np.zeros(0), posre_fb_atoms=np.zeros(0, np.int32),
+        posre_fb_x0=np.zeros((0, 3)), posre_fb_hk=np.zeros(0), posre_fb_tol=np.zeros(0),
+        posre_h_atoms=np.zeros(0, np.int32), posre_h_x0=np.zeros((0, 3)),
+        posre_h_hk=np.zeros((0, 3)), exception_pairs=np.zeros((0, 2), np.int32),
+        exception_qq=np.zeros(0), exception_sigma=np.zeros(0), exception_epsilon=np.zeros(0))
+    e1, _ = OT.energy(x, only_torsions, ff.box)
+    phi = OT.dihedral(*[x[i] for i in (3, 4, 5, 6)])
+    want = 1.5 * KCAL * math.cos(phi - math.pi) + 0.2 * KCAL * math.cos(3 * phi - math.pi)
+    assert e1 == pytest.approx(want, rel=1e-12, abs=1e-12)
+    s = to_sdm_system(ff, x)
+    assert s.lj_rule == 1
+    assert to_sdm_system(d.createSystem(nonbondedMethod="CutoffPeriodic"), x).lj_rule == 0
 
 
 def test_multi_file_offsets(tmp_path):

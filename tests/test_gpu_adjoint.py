@@ -54,6 +54,38 @@ def test_u_and_grad_match_oracle(c4):
         assert np.allclose(got, ref, rtol=1e-9, atol=1e-9 * np.max(np.abs(ref)))
 
 
+def test_geometric_rule_u_and_grad_match_oracle(hip_device):
+    """The OPLS (geometric sigma) rule: sigma_ij = sqrt(s_i s_j), so d sigma_ij / d s_i =
+    s_j / (2 sigma_ij); u and its parameter gradient against the oracle (whose derivative
+    is pinned by central differences, tests/test_oracle_pins.py)."""
+    frames = [systems.make_small(frame=k) for k in range(3)]
+    for f in frames:
+        f.lj_rule = 1
+    s = frames[0]
+    print(f"case geometric adjoint small n={s.n} frames={len(frames)}", flush=True)
+    p = systems.params_config1()
+    model = adjoint.AdjointModel(s, s.ligand, p)
+    X = frames_tensor(frames)
+    q = torch.tensor(s.charge[s.ligand], dtype=torch.float64, device="cuda", requires_grad=True)
+    sg = torch.tensor(s.sigma[s.ligand], dtype=torch.float64, device="cuda", requires_grad=True)
+    ep = torch.tensor(s.epsilon[s.ligand], dtype=torch.float64, device="cuda", requires_grad=True)
+    u = adjoint.PerturbationEnergy.apply(q, sg, ep, X, model)
+    u.sum().backward()
+    topo = O.Topology(s)
+    gs = np.zeros(len(s.ligand))
+    for k, f in enumerate(frames):
+        _, uref, *_ = O.sdm_direct(topo, f.positions)
+        assert math.isclose(float(u[k]), uref, rel_tol=1e-10, abs_tol=1e-8)
+        gs += O.du_dparams(topo, f.positions, s.ligand)[1]
+    got = sg.grad.cpu().numpy()
+    assert np.allclose(got, gs, rtol=1e-9, atol=1e-9 * np.max(np.abs(gs)))
+    # the arithmetic rule's gradient differs
+    s0 = s.with_coordinates(s.positions)
+    s0.lj_rule = 0
+    g0 = sum(O.du_dparams(O.Topology(s0), f.positions, s.ligand)[1] for f in frames)
+    assert np.max(np.abs(g0 - gs)) > 1e-3 * np.max(np.abs(gs))
+
+
 def _oracle_mean_W(s, frames, p, q=None, sig=None, eps=None, alpha=None, u0=None, w0=None):
     s2 = s.with_coordinates(s.positions)
     s2.charge = s.charge.copy() if q is None else q

@@ -111,6 +111,29 @@ def test_small_components(hip_device, frame, kernel):
     check_components(s, kernel=kernel)
 
 
+@pytest.mark.parametrize("kernel", ["block_n3", "block_n3_split4", "block_n3_split16"])
+@pytest.mark.parametrize("frame", [0, 1, 2])
+def test_geometric_rule_components(hip_device, frame, kernel):
+    """The OPLS rule (sigma_ij = sqrt(s_i s_j), the reference's CustomNonbondedForce for
+    OPLS, desmonddmsfile75.py:781) on the block Newton-3 kernel, against the oracle."""
+    s = systems.make_small(frame=frame)
+    s.lj_rule = 1
+    check_components(s, kernel=kernel)
+    s = systems.make_config1(frame=frame)
+    s.lj_rule = 1
+    check_components(s, kernel=kernel)
+
+
+def test_geometric_rule_other_kernels_refused(hip_device):
+    s = systems.make_small()
+    s.lj_rule = 1
+    ctx = _native.NativeContext(s, 1)
+    for kernel in ("block_full", "grp16", "grp8"):
+        with pytest.raises(_native.SDMError):
+            ctx.set_tuning(**KERNELS[kernel])
+    assert ctx.autotune(5)["pair_newton3"] == 1 and ctx.get_tuning()["pair_group"] == 0
+
+
 @pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("frame", [0, 1])
 def test_config1_components(hip_device, frame, kernel):

@@ -12,6 +12,10 @@ friction 0.1/ps, 1 fs.  The steps run as example/test.py:198-212 does (step(n) i
 chunks, then getPotEnergy / getBindE), checked against the FP64 oracle (group 2 at x and
 x+d, bonded, DMS terms, restraints, scalar stage) at the positions the last step of each
 chunk evaluated.  Tolerances: energies 1e-5 relative, forces 1e-4 RMS relative.
+Both tests run the file as example/test.py reads it and as createSystem(OPLS=True) reads
+it (tests/golden/oa_g6_g3_opls.npz): its nonbonded_info declares the geometric rule, and
+with OPLS the reference's reader moves LJ to a geometric-sigma CustomNonbondedForce and
+writes torsions as f cos(n phi - phi0) (desmonddmsfile75.py:652-656, :780-801).
 """
 import dataclasses
 import os
@@ -38,8 +42,8 @@ def rms_rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
 
 
-def load_oa():
-    z = np.load(os.path.join(HERE, "golden", "oa_g6_g3.npz"))
+def load_oa(opls=False):
+    z = np.load(os.path.join(HERE, "golden", "oa_g6_g3_opls.npz" if opls else "oa_g6_g3.npz"))
     names = [f.name for f in dataclasses.fields(DMSForceField)]
     kw = {k: z[k] for k in names if k in z.files}
     cutoff, opls, cm = z["scalars"]
@@ -101,10 +105,11 @@ def oracle_eval(s, ff, x):
     return E1 + e2, E2 + e2, Eb + e1 + er, F1 + f2, F2 + f2, Fb + f1 + fr
 
 
-def test_vacuum_components_match_oracle(hip_device):
-    s, ff, lig1, lig2 = load_oa()
-    assert not s.periodic and ff.constraints.shape[0] == 86
-    print(f"case oa-g6-g3 n={s.n} non-periodic cutoff={s.cutoff}", flush=True)
+@pytest.mark.parametrize("opls", [False, True])
+def test_vacuum_components_match_oracle(hip_device, opls):
+    s, ff, lig1, lig2 = load_oa(opls)
+    assert not s.periodic and ff.constraints.shape[0] == 86 and s.lj_rule == int(opls)
+    print(f"case oa-g6-g3 n={s.n} non-periodic cutoff={s.cutoff} opls={opls}", flush=True)
     ctx = _native.NativeContext(s, 1)
     integ = make_integrator(s, lig1, lig2)
     ctx.set_params(0, integ._p)
@@ -129,11 +134,13 @@ def test_vacuum_components_match_oracle(hip_device):
     assert abs(e2["U0"] - E1) <= 1e-5 * max(1.0, abs(E1)) and e2["flags"] == 0
 
 
-def test_vacuum_facade_steps_match_oracle(hip_device):
+@pytest.mark.parametrize("opls", [False, True])
+def test_vacuum_facade_steps_match_oracle(hip_device, opls):
     """example/test.py's loop through the facade: step(n) in chunks, then getPotEnergy()
     and getBindE(), each against the oracle at the positions the chunk's last step
     evaluated (the record a step returns is that of the positions before its update)."""
-    s, ff, lig1, lig2 = load_oa()
+    s, ff, lig1, lig2 = load_oa(opls)
+    print(f"case oa-g6-g3 facade steps opls={opls}", flush=True)
     integ = make_integrator(s, lig1, lig2)
     ctx = Context(s, integ)
     assert integ.getConstraintTolerance() == 1e-5

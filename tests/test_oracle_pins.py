@@ -173,7 +173,7 @@ def test_nonequilibrium_schedule_and_work_by_fd():
 
 
 # ------------------------------------------------------------- nonbonded ----
-def _two_atom(q1, q2, s1, s2, e1, e2, r, rc=1.0, eps_rf=78.5, box=10.0):
+def _two_atom(q1, q2, s1, s2, e1, e2, r, rc=1.0, eps_rf=78.5, box=10.0, lj_rule=0):
     s = systems.make_small()
     sys2 = s.with_coordinates(np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]]))
     sys2.charge = np.array([q1, q2])
@@ -188,6 +188,7 @@ def _two_atom(q1, q2, s1, s2, e1, e2, r, rc=1.0, eps_rf=78.5, box=10.0):
     sys2.box = np.array([box, box, box])
     sys2.cutoff = rc
     sys2.eps_rf = eps_rf
+    sys2.lj_rule = lj_rule
     topo = O.Topology(sys2)
     E, F, n = O.nonbonded(topo, sys2.positions, "brute")
     return E, F, n
@@ -207,6 +208,22 @@ def test_lorentz_berthelot_mixing():
     s1, s2, e1, e2 = 0.25, 0.35, 0.2, 0.8
     E, F, _ = _two_atom(0, 0, s1, s2, e1, e2, 2 ** (1 / 6) * 0.5 * (s1 + s2))
     assert math.isclose(E, -math.sqrt(e1 * e2), rel_tol=1e-12) and np.max(np.abs(F)) < 1e-9
+
+
+def test_geometric_mixing():
+    """OPLS (the reference's CustomNonbondedForce for OPLS, desmonddmsfile75.py:781):
+    sigma12 = sqrt(s1 s2), so the minimum -sqrt(e1 e2) sits at 2^(1/6) sqrt(s1 s2), not at
+    the Lorentz-Berthelot distance."""
+    s1, s2, e1, e2 = 0.25, 0.35, 0.2, 0.8
+    rg = 2 ** (1 / 6) * math.sqrt(s1 * s2)
+    E, F, _ = _two_atom(0, 0, s1, s2, e1, e2, rg, lj_rule=1)
+    assert math.isclose(E, -math.sqrt(e1 * e2), rel_tol=1e-12) and np.max(np.abs(F)) < 1e-9
+    E, _, _ = _two_atom(0, 0, s1, s2, e1, e2, 2 ** (1 / 6) * 0.5 * (s1 + s2), lj_rule=1)
+    assert E > -math.sqrt(e1 * e2) * (1 - 1e-4)
+    # equal sigmas: both rules agree
+    Ea, _, _ = _two_atom(0, 0, 0.3, 0.3, e1, e2, 0.41, lj_rule=0)
+    Eg, _, _ = _two_atom(0, 0, 0.3, 0.3, e1, e2, 0.41, lj_rule=1)
+    assert math.isclose(Ea, Eg, rel_tol=1e-14)
 
 
 def test_reaction_field_special_cases():
@@ -362,8 +379,10 @@ def test_force_mix_linear_endpoints(small):
         assert np.array_equal(F, ref)
 
 
-def test_du_dparams_by_fd():
+@pytest.mark.parametrize("lj_rule", [0, 1])
+def test_du_dparams_by_fd(lj_rule):
     s = systems.make_small(n_water=60, n_lig=5, frame=4, displ=(0.7, 0.2, 0.0))
+    s.lj_rule = lj_rule
     topo = O.Topology(s)
     sel = s.ligand[:3].astype(np.int32)
     dq, dsig, deps = O.du_dparams(topo, s.positions, sel)
