@@ -115,8 +115,10 @@ __device__ __forceinline__ int pair64(double dx, double dy, double dz, double qq
         *fr = 0.0;
         return 0;
     }
-    double ir2 = 1.0 / r2;
-    double ir = sqrt(ir2);
+    // rsqrt (the device library's FP64 reciprocal square root, refined to full precision)
+    // in place of an IEEE division and a square root: a third fewer instructions per call
+    double ir = rsqrt(r2);
+    double ir2 = ir * ir;
     double s2 = sig * sig * ir2;
     double s6 = s2 * s2 * s2;
     *e = eps4 * s6 * (s6 - 1.0) + qq * (ir + krf * r2 - crf);
