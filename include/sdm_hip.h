@@ -178,10 +178,12 @@ typedef struct {
                                      the tile-list build as well), 2 on the main stream
                                      between the tile lists and the pair kernel, 3 as 1
                                      but with the displaced-state chain joined back
-                                     before the pair kernel (beside the tile lists only).
-                                     With 0, 1 and 3 the bonds and angles start with the
-                                     step on a side stream (beside the spatial sort);
-                                     with 2 they run with the rest.  Same results. */
+                                     before the pair kernel (beside the tile lists only),
+                                     4 as 1 but with the bonds and angles forked with the
+                                     rest.  With 0, 1 and 3 the bonds and angles start
+                                     with the step on a side stream (beside the spatial
+                                     sort, one more fork event per step); with 2 they
+                                     run with the rest.  Same results. */
     float list_skin;              /* Verlet skin of the tile lists in nm, 0 to
                                      SDM_MAX_LIST_SKIN (block pair kernels only; 0, the
                                      default, builds them on every evaluation).  Lists
@@ -420,7 +422,7 @@ int sdm_time_pair_kernel(sdm_context* ctx, int32_t nsteps, double* avg_ms);
 /* Launch-configuration search (project-written autotuner): times each candidate (pair
    kernel, waves per workgroup and per i-block, sort grid) on the context's own replicas
    for `steps` steps, then times the three fastest, each with and without graph replay
-   and with each of the four placements of the side work (side_fork), for 3 x `steps`
+   and with each of the five placements of the side work (side_fork), for 3 x `steps`
    steps in two rounds, and keeps the one with the smallest total; with a block pair
    kernel it then times that one with list_skin 0, 0.01, 0.02, 0.03 and 0.04 nm the same
    way.  The state the context had before the call is restored. */

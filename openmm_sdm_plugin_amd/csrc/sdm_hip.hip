@@ -551,9 +551,11 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     // 2: it runs on the main stream between the tile lists and the pair kernel (no fork;
     // while the pair kernel fills the GPU, side kernels beside it wait for free wave
     // slots and can outlast it), 3: as 1, but the displaced-state chain joins the main
-    // stream before the pair kernel (it runs beside the tile lists only).  The bonded
-    // terms need only the positions and write their own FP64 array: with a fork (0, 1,
-    // 3) they start with the step, beside the sort.
+    // stream before the pair kernel (it runs beside the tile lists only), 4: as 1, but the
+    // bonded terms fork with the rest (no event at the start of the step).  The bonded
+    // terms need only the positions and write their own FP64 array: with 0, 1 and 3 they
+    // start with the step, beside the sort (measured: 22 replicas 0.7% faster than 4; one
+    // replica 1.6% slower replayed from a graph, 4% launched directly).
     const int mode = c->tune.side_fork;
     hipStream_t s = c->stream;
     hipStream_t s1 = mode == 2 ? s : c->side[0], s2 = mode == 2 ? s : c->side[1];
@@ -565,18 +567,19 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
                 c->bond_k, c->ang_atoms, c->ang_t0, c->ang_k, c->x64, c->Fb, c->e_bond,
                 c->nbw, c->flags);
     };
-    if (mode != 2) {
+    if (mode != 2 && mode != 4) {
         (void)hipEventRecord(c->ev_fork0, s);
         (void)hipStreamWaitEvent(s1, c->ev_fork0, 0);
         bonded();
     }
     enqueue_order(c, full, energy);
-    const bool tiles_after_fork = mode == 1 || mode == 3;
+    const bool tiles_after_fork = mode == 1 || mode == 3 || mode == 4;
     if (!tiles_after_fork) enqueue_tiles(c, energy);
     if (mode != 2) {
         (void)hipEventRecord(c->ev_fork, s);
         (void)hipStreamWaitEvent(s1, c->ev_fork, 0);
         (void)hipStreamWaitEvent(s2, c->ev_fork, 0);
+        if (mode == 4) bonded();
     }
     if (tiles_after_fork) enqueue_tiles(c, energy);
     if (mode == 2) bonded();
@@ -620,7 +623,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
         (void)hipEventRecord(c->ev_join[0], s1);
         (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
     }
-    if (mode == 0 || mode == 1) {
+    if (mode == 0 || mode == 1 || mode == 4) {
         (void)hipEventRecord(c->ev_join[1], s2);
         (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
     }
@@ -1871,8 +1874,8 @@ int sdm_set_tuning(sdm_context* c, const sdm_tuning* t) {
         return fail(SDM_ERR_INVALID, "pair_split must be 1, 2, 4, 8, 16 or 32");
     if (t->use_graph != 0 && t->use_graph != 1)
         return fail(SDM_ERR_INVALID, "use_graph must be 0 or 1");
-    if (t->side_fork < 0 || t->side_fork > 3)
-        return fail(SDM_ERR_INVALID, "side_fork must be 0, 1, 2 or 3");
+    if (t->side_fork < 0 || t->side_fork > 4)
+        return fail(SDM_ERR_INVALID, "side_fork must be 0, 1, 2, 3 or 4");
     if (!(t->list_skin >= 0.f && t->list_skin <= (float)SDM_MAX_LIST_SKIN))
         return fail(SDM_ERR_INVALID, "list_skin must be in [0, 0.1] nm");
     if (c->g.geo && (t->pair_group != 0 || t->pair_newton3 != 1))
@@ -1965,7 +1968,7 @@ int sdm_autotune(sdm_context* c, int32_t steps, sdm_tuning* chosen) {
                      [](const auto& p, const auto& q) { return p.first < q.first; });
     std::vector<sdm_tuning> finalists;
     for (size_t k = 0; k < timed.size() && k < 3; k++)
-        for (int variant = 0; variant < 8; variant++) {
+        for (int variant = 0; variant < 10; variant++) {
             sdm_tuning t = timed[k].second;
             t.use_graph = variant & 1;
             t.side_fork = variant >> 1;

@@ -460,7 +460,7 @@ def test_forces_only_steps_match_energy_steps_on_ladder(hip_device, use_graph, s
 @pytest.mark.parametrize("field,value", [("pair_newton3", 2), ("use_graph", 2),
                                          ("pair_group", 4), ("pair_waves_per_group", 3),
                                          ("sort_interval", 0), ("energy_every_step", -1),
-                                         ("pair_split", 3), ("side_fork", 4),
+                                         ("pair_split", 3), ("side_fork", 5),
                                          ("list_skin", 0.2), ("list_skin", -0.01)])
 def test_tuning_out_of_range_refused(hip_device, field, value):
     """sdm_set_tuning refuses values outside each field's documented set and keeps the
@@ -482,7 +482,7 @@ def test_side_work_placement_gives_same_bits(hip_device, use_graph):
     frames = [systems.make_config1(frame=k) for k in range(3)]
     params = systems.params_config3(3)
     xs = []
-    for fork in (0, 1, 2, 3):
+    for fork in (0, 1, 2, 3, 4):
         print(f"case side_fork={fork} use_graph={use_graph}", flush=True)
         ctx = _native.NativeContext(frames[0], 3)
         ctx.set_tuning(side_fork=fork, use_graph=use_graph, pair_split=4)
