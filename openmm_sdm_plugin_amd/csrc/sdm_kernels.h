@@ -2394,7 +2394,7 @@ __global__ __launch_bounds__(LIGCAND_THREADS) void k_ligcand(
     const double* __restrict__ cdispl, const double* __restrict__ x64,
     const float4* __restrict__ bb_lo, const float4* __restrict__ bb_hi, int maxc,
     int* __restrict__ cand, int* __restrict__ ncand, float list_r2, Geom g,
-    int* __restrict__ flags) {
+    int* __restrict__ flags, int* __restrict__ cmap) {
     constexpr int NW = LIGCAND_THREADS / 64;
     __shared__ float4 blo[2 * SDM_MAX_CLASSES], bhi[2 * SDM_MAX_CLASSES];
     __shared__ double red[6][NW];
@@ -2453,6 +2453,9 @@ __global__ __launch_bounds__(LIGCAND_THREADS) void k_ligcand(
         for (int w = 0; w < wv; w++) before += wcount[w];
         int pos = before + lane_rank(m);
         if (ok && pos < maxc) out[pos] = jb;
+        // block -> candidate index (-1: none), for the Langevin kernel's gather of the
+        // displaced-state forces
+        if (cmap && jb < NB) cmap[(size_t)r * NB + jb] = (ok && pos < maxc) ? pos : -1;
         for (int w = 0; w < NW; w++) count += wcount[w];
         __syncthreads();  // wcount is rewritten next round
     }
@@ -3025,21 +3028,56 @@ __global__ __launch_bounds__(64) void k_mix(int N, int Np, int L, int maxc,
 // fixed-point accumulator (slot order) plus, when there are bonded terms, their FP64
 // forces Fb (atom order).  Atoms with a constraint (cflag) leave their displacement
 // dt*v in dxs for k_constrain.
+// MixGather: k_mix's work done here instead, per atom (one launch fewer per step): the
+// displaced-state force sp * dF of atom i, from the ligand sums (lig_of: the atom's
+// ligand index, -1 for environment atoms) or from its block's environment sums (cmap:
+// block -> candidate index of this step, -1 for none), added to the accumulator as k_mix
+// adds it, so the accumulator holds the same bits afterwards.
+struct MixGather {
+    const int* lig_of;
+    const int* cmap;
+    const double4* dB;
+    const double4* dlig;
+    int NB, maxc, L;
+};
+
 __global__ __launch_bounds__(256) void k_langevin(int N, int Np, const double* __restrict__ invm,
-                                                  const long long* __restrict__ F,
+                                                  long long* __restrict__ F,
                                                   const double* __restrict__ Fb,
                                                   double* __restrict__ x64, double* __restrict__ v64,
                                                   const DevRecord* __restrict__ rec,
                                                   const DevState* __restrict__ state,
                                                   const int* __restrict__ slot_of,
                                                   const int* __restrict__ cflag,
-                                                  double* __restrict__ dxs) {
+                                                  double* __restrict__ dxs, MixGather mg,
+                                                  int* __restrict__ flags) {
     int r = blockIdx.y;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    const DevRecord& Rr = rec[r];
+    if (mg.lig_of) {
+        const int li = mg.lig_of[i];
+        const int sl = slot_of[(size_t)r * Np + i];
+        const double4* dp = nullptr;
+        if (li >= 0) {
+            dp = mg.dlig + (size_t)r * mg.L + li;
+        } else {
+            const int cb = mg.cmap[(size_t)r * mg.NB + (sl >> 6)];
+            if (cb >= 0) dp = mg.dB + ((size_t)r * mg.maxc + cb) * 64 + (sl & 63);
+        }
+        if (dp) {
+            const double4 d = *dp;
+            const double sp = Rr.sp;
+            long long* Fm = F + (size_t)r * 3 * Np;
+            int bad = 0;
+            Fm[sl] += to_fixed(sp * d.x, &bad);
+            Fm[Np + sl] += to_fixed(sp * d.y, &bad);
+            Fm[2 * Np + sl] += to_fixed(sp * d.z, &bad);
+            if (bad) atomicOr(&flags[r], 2);
+        }
+    }
     double im = invm[i];
     if (im == 0.0) return;
-    const DevRecord& Rr = rec[r];
     double vscale = Rr.vscale, fscale = Rr.fscale, noise = Rr.noisescale, dt = Rr.dt;
     double sim = sqrt(im);
     double gz[3];
