@@ -1923,9 +1923,10 @@ __global__ __launch_bounds__(256) void k_bonded(
             int j = bond_atoms[2 * t + (role ^ 1)];
             double d[3];
             for (int a = 0; a < 3; a++) d[a] = X[a * Np + j] - xi[a];
-            double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            double dr = rr - bond_r0[t];
-            double gg = rr > 0.0 ? bond_k[t] * dr / rr : 0.0;
+            const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double irr = rsq > 0.0 ? rsqrt(rsq) : 0.0;
+            double dr = rsq * irr - bond_r0[t];
+            double gg = bond_k[t] * dr * irr;
             for (int a = 0; a < 3; a++) f[a] += gg * d[a];  // pulls i toward j when stretched
             if (role == 0) e += 0.5 * bond_k[t] * dr * dr;
         }
@@ -1938,17 +1939,24 @@ __global__ __launch_bounds__(256) void k_bonded(
                 v1[a] = X[a * Np + ia] - xm;
                 v2[a] = X[a * Np + ic] - xm;
             }
-            double r1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-            double r2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
-            double cs = (v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]) / (r1 * r2);
-            cs = fmin(fmax(cs, -1.0), 1.0);
-            double th = acos(cs);
+            // theta = atan2(|v1 x v2|, v1 . v2): one call gives the angle and, with the
+            // reciprocal lengths, its sine (acos plus sin took twice the instructions and
+            // is worse conditioned near 0 and pi)
+            const double i1 = rsqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+            const double i2 = rsqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+            const double dot = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+            const double cx = v1[1] * v2[2] - v1[2] * v2[1], cy = v1[2] * v2[0] - v1[0] * v2[2],
+                         cz = v1[0] * v2[1] - v1[1] * v2[0];
+            const double cr = sqrt(cx * cx + cy * cy + cz * cz);
+            double th = atan2(cr, dot);
+            const double cs = dot * i1 * i2, sn = cr * i1 * i2;
             double dth = th - ang_t0[k];
-            double gg = ang_k[k] * dth / fmax(sin(th), 1e-12);
+            double gg = ang_k[k] * dth / fmax(sn, 1e-12);
             if (role == 0) e += 0.5 * ang_k[k] * dth * dth;
+            const double p12 = gg * i1 * i2, p11 = gg * cs * i1 * i1, p22 = gg * cs * i2 * i2;
             for (int a = 0; a < 3; a++) {
-                double fa = gg * (v2[a] / (r1 * r2) - cs * v1[a] / (r1 * r1));
-                double fc = gg * (v1[a] / (r1 * r2) - cs * v2[a] / (r2 * r2));
+                double fa = p12 * v2[a] - p11 * v1[a];
+                double fc = p12 * v1[a] - p22 * v2[a];
                 f[a] += role == 0 ? fa : (role == 2 ? fc : -(fa + fc));
             }
         }
