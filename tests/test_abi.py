@@ -34,6 +34,9 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(_native.sdm_params) == 8 + 8 * 11 + 8 + 8 * 13 + 8 + 8
     assert C.sizeof(_native.sdm_energies) == 8 * 17 + 8 + 8
     assert C.sizeof(_native.sdm_tuning) == 40
+    # sdm_system_desc: ints padded to the following pointers/doubles; lj_rule last
+    assert C.sizeof(_native.sdm_system_desc) == 184
+    assert _native.sdm_system_desc.lj_rule.offset == 176
     # 2 ints, 4 pointers, 5 doubles, 8 ints, 10 doubles
     assert C.sizeof(_native.sdm_centroid_restraint) == 8 + 32 + 40 + 32 + 80
     assert C.sizeof(_native.sdm_alignment_restraint) == 24 + 48
