@@ -48,6 +48,34 @@ void System::addAngle(int a, int b, int c, double theta0, double k) {
     ang_k_.push_back(k);
 }
 
+void System::setVirtualSite(int index, VirtualSite* site) {
+    std::shared_ptr<const VirtualSite> owned(site);  // owned before anything can throw
+    if (index < 0 || index >= (int)mass_.size()) throw SDMException("particle index out of range");
+    if (!site) throw SDMException("null virtual site");
+    vsites_[index] = std::move(owned);
+}
+
+const VirtualSite& System::getVirtualSite(int index) const {
+    auto it = vsites_.find(index);
+    if (it == vsites_.end()) throw SDMException("This particle is not a virtual site");
+    return *it->second;
+}
+
+int VirtualSite::getParticle(int i) const {
+    if (i < 0 || i >= (int)particles_.size()) throw SDMException("particle index out of range");
+    return particles_[i];
+}
+
+double TwoParticleAverageSite::getWeight(int i) const {
+    if (i < 0 || i > 1) throw SDMException("weight index out of range");
+    return w_[i];
+}
+
+double ThreeParticleAverageSite::getWeight(int i) const {
+    if (i < 0 || i > 2) throw SDMException("weight index out of range");
+    return w_[i];
+}
+
 // --------------------------------------------------------- SDMParameters ----
 // Defaults of the reference constructor, LangevinIntegratorSDM.cpp:48-85.
 SDMParameters::SDMParameters(int nParticles) {
@@ -212,7 +240,25 @@ void Context::create(const std::vector<double>& displacement) {
     d.periodic = sys_.periodic_ ? 1 : 0;
     d.cutoff = sys_.cutoff_;
     d.eps_rf = sys_.eps_rf_;
+    d.lj_rule = sys_.lj_rule_;
     check(sdm_create(&d, 1, &h_));
+    if (sys_.vsites_.empty()) return;
+    std::vector<int32_t> kind, atoms;
+    std::vector<double> w;
+    for (const auto& [index, site] : sys_.vsites_) {
+        kind.push_back(site->kind_);
+        atoms.push_back(index);
+        for (int k = 0; k < 3; k++)
+            atoms.push_back(k < site->getNumParticles() ? site->particles_[k] : -1);
+        w.insert(w.end(), site->w_.begin(), site->w_.end());
+    }
+    int rc = sdm_set_virtual_sites(h_, (int32_t)kind.size(), kind.data(), atoms.data(), w.data());
+    if (rc != SDM_OK) {
+        std::string msg = sdm_last_error();
+        sdm_destroy(h_);  // the constructors' own clean-up covers only what follows create()
+        h_ = nullptr;
+        throw SDMException(msg);
+    }
 }
 
 void Context::setPositions(const std::vector<Vec3>& x) {

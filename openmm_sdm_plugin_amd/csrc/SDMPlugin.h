@@ -17,6 +17,7 @@
 #pragma once
 
 #include <array>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,50 @@ public:
 
 // Particles + the group-2 (nonbonded LJ + reaction field) and group-1 (harmonic bonds
 // and angles) terms of an OpenMM System.
+// OpenMM 7's VirtualSite classes, the three the reference's DMS reader attaches
+// (example/desmonddmsfile75.py:877, :892, :907).  A site is a massless particle placed from
+// its parents after every step; the forces on it move to the parents.
+class VirtualSite {
+public:
+    virtual ~VirtualSite() = default;
+    int getNumParticles() const { return (int)particles_.size(); }
+    int getParticle(int i) const;
+
+protected:
+    friend class Context;
+    VirtualSite(int kind, std::vector<int> particles, const Vec3& w)
+        : kind_(kind), particles_(std::move(particles)), w_(w) {}
+    int kind_;                    // SDM_VSITE_*
+    std::vector<int> particles_;  // parents
+    Vec3 w_;
+};
+
+// x = w1 x1 + w2 x2
+class TwoParticleAverageSite : public VirtualSite {
+public:
+    TwoParticleAverageSite(int p1, int p2, double w1, double w2)
+        : VirtualSite(SDM_VSITE_AVERAGE2, {p1, p2}, {{w1, w2, 0.0}}) {}
+    double getWeight(int i) const;
+};
+
+// x = w1 x1 + w2 x2 + w3 x3
+class ThreeParticleAverageSite : public VirtualSite {
+public:
+    ThreeParticleAverageSite(int p1, int p2, int p3, double w1, double w2, double w3)
+        : VirtualSite(SDM_VSITE_AVERAGE3, {p1, p2, p3}, {{w1, w2, w3}}) {}
+    double getWeight(int i) const;
+};
+
+// x = x1 + w12 r12 + w13 r13 + wCross (r12 x r13), r1k = xk - x1
+class OutOfPlaneSite : public VirtualSite {
+public:
+    OutOfPlaneSite(int p1, int p2, int p3, double w12, double w13, double wCross)
+        : VirtualSite(SDM_VSITE_OUT_OF_PLANE, {p1, p2, p3}, {{w12, w13, wCross}}) {}
+    double getWeight12() const { return w_[0]; }
+    double getWeight13() const { return w_[1]; }
+    double getWeightCross() const { return w_[2]; }
+};
+
 class System {
 public:
     int addParticle(double mass);
@@ -51,6 +96,18 @@ public:
     void setCutoffDistance(double rc) { cutoff_ = rc; }
     void setReactionFieldDielectric(double e) { eps_rf_ = e; }
     double getCutoffDistance() const { return cutoff_; }
+    // OpenMM's System::setVirtualSite: the System takes ownership of site; the particle
+    // must have mass 0, its parents mass.  Checked when a Context is created.
+    void setVirtualSite(int index, VirtualSite* site);
+    bool isVirtualSite(int index) const { return vsites_.count(index) != 0; }
+    const VirtualSite& getVirtualSite(int index) const;
+    // How the LJ sigma of a pair combines: LorentzBerthelot, (s_i + s_j)/2 as OpenMM's
+    // NonbondedForce (the default), or Geometric, sqrt(s_i s_j) as the OPLS
+    // CustomNonbondedForce of the reference's DMS reader (desmonddmsfile75.py:780-793);
+    // epsilon is sqrt(e_i e_j) either way.
+    enum CombiningRule { LorentzBerthelot = SDM_LJ_ARITHMETIC, Geometric = SDM_LJ_GEOMETRIC };
+    void setCombiningRule(CombiningRule r) { lj_rule_ = r; }
+    CombiningRule getCombiningRule() const { return lj_rule_; }
 
 private:
     friend class Context;
@@ -62,6 +119,8 @@ private:
     Vec3 box_{{0, 0, 0}};
     double cutoff_ = 1.0, eps_rf_ = 78.3;
     bool periodic_ = true;
+    CombiningRule lj_rule_ = LorentzBerthelot;
+    std::map<int, std::shared_ptr<const VirtualSite>> vsites_;  // shared: Contexts copy
 };
 
 class LangevinIntegratorSDM;

@@ -50,6 +50,24 @@ int main(int argc, char** argv) {
         in >> a >> b >> c >> t0 >> kk;
         sys.addAngle(a, b, c, t0, kk);
     }
+    // optional tail: the combining rule, then virtual sites as
+    // "kind site p1 p2 p3 w1 w2 w3" (kind 0: two-particle average, p3 unused; 1: three-
+    // particle average; 2: out of plane)
+    int lj_rule = 0, nvs = 0;
+    if (in >> lj_rule >> nvs) {
+        sys.setCombiningRule(lj_rule ? System::Geometric : System::LorentzBerthelot);
+        for (int k = 0; k < nvs; k++) {
+            int kind, s, p1, p2, p3;
+            double w1, w2, w3;
+            in >> kind >> s >> p1 >> p2 >> p3 >> w1 >> w2 >> w3;
+            if (kind == 0)
+                sys.setVirtualSite(s, new TwoParticleAverageSite(p1, p2, w1, w2));
+            else if (kind == 1)
+                sys.setVirtualSite(s, new ThreeParticleAverageSite(p1, p2, p3, w1, w2, w3));
+            else
+                sys.setVirtualSite(s, new OutOfPlaneSite(p1, p2, p3, w1, w2, w3));
+        }
+    }
     sys.setPeriodicBoxVectors({bx, by, bz});
     sys.setCutoffDistance(rc);
     sys.setReactionFieldDielectric(erf);
