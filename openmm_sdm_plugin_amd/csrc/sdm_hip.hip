@@ -79,6 +79,7 @@ struct sdm_context {
     // run concurrently with the pair kernel
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_fork0 = nullptr, ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipEvent_t ev_tiles = nullptr;  // the tile lists are built (energy evaluations)
     int N = 0, Np = 0, NB = 0, R = 0, L = 0, LB = 0, ncls = 0;
     // 0: CutoffNonPeriodic (no images; the sort grid spans reg_lo .. reg_hi, the union of
     // the positions' bounding boxes over the replicas, widened by GRID_MARGIN)
@@ -433,11 +434,11 @@ void enqueue_sort(sdm_context* c, bool full) {
 }
 
 // FP64 pair energies into e64 (needs the tile lists and FP64 records of this evaluation)
-void enqueue_energy64(sdm_context* c) {
+void enqueue_energy64(sdm_context* c, hipStream_t st) {
     const int half = c->tune.pair_group != 0 ? 1 : c->tune.pair_newton3;
     unsigned waves = (unsigned)(c->R * c->NB * E64_SPLIT);
     size_t lds = (size_t)std::max(1, c->max_ex) * 64 * sizeof(int);
-    k_energy64<<<waves, 64, lds, c->stream>>>(c->R, c->N, c->Np, c->NB, c->maxj, E64_SPLIT,
+    k_energy64<<<waves, 64, lds, st>>>(c->R, c->N, c->Np, c->NB, c->maxj, E64_SPLIT,
                                               half, c->arec, c->rec64, c->orig_s, c->slot_of,
                                               c->tiles, c->tmask, c->ntiles, c->bb_lo,
                                               c->bb_hi, c->ex_start, c->ex_list, c->max_ex,
@@ -621,6 +622,14 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
             c->sig64, c->eps64, c->ex_start, c->ex_list, c->x64, c->ncand, c->maxc, c->dA,
             c->dlig, c->u_part, c->g, nullptr);
     }
+    // the FP64 pair energies need the tile lists and this evaluation's FP64 records, not
+    // the pair forces: they run beside the pair kernel on the first side stream, which
+    // joins before the scalar stage
+    if (energy && mode != 2) {
+        (void)hipEventRecord(c->ev_tiles, s);
+        (void)hipStreamWaitEvent(s1, c->ev_tiles, 0);
+        enqueue_energy64(c, s1);
+    }
     if (mode == 3) {
         (void)hipEventRecord(c->ev_join[1], s2);
         (void)hipStreamWaitEvent(s, c->ev_join[1], 0);
@@ -629,7 +638,7 @@ void enqueue_eval(sdm_context* c, bool advance, bool components, bool full, bool
     // forces in FP32; the pair energies, when asked for, in FP64 by their own kernel
     enqueue_pair(c, false, false);
     if (pe1) (void)hipEventRecord(pe1, s);
-    if (energy) enqueue_energy64(c);
+    if (energy && mode == 2) enqueue_energy64(c, s);
     if (mode != 2) {
         (void)hipEventRecord(c->ev_join[0], s1);
         (void)hipStreamWaitEvent(s, c->ev_join[0], 0);
@@ -894,7 +903,8 @@ int sdm_create(const sdm_system_desc* sys, int32_t n_replicas, sdm_context** out
         hipEventCreateWithFlags(&c->ev_fork0, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_tiles, hipEventDisableTiming) != hipSuccess)
         return bail(fail(SDM_ERR_HIP, "hipStreamCreate/hipEventCreate failed"));
     c->N = N;
     c->max_ex = max_ex;
@@ -1055,7 +1065,7 @@ int sdm_destroy(sdm_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (void* p : c->allocs) (void)hipFree(p);
-    for (hipEvent_t e : {c->ev_fork0, c->ev_fork, c->ev_join[0], c->ev_join[1]})
+    for (hipEvent_t e : {c->ev_fork0, c->ev_fork, c->ev_join[0], c->ev_join[1], c->ev_tiles})
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t st : {c->side[0], c->side[1]})
         if (st) (void)hipStreamDestroy(st);
