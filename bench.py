@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--profile-large", type=int, default=0,
                     help="profiling mode: N eager steps of the 8-replica 99,920-atom ladder "
                          "(its benchmark tuning, kernels launched one by one), no other legs")
+    ap.add_argument("--profile-single", type=int, default=0,
+                    help="profiling mode: N steps of one replica of the 22,750-atom system "
+                         "replayed from captured graphs (32 waves per i-block, side work "
+                         "forked after the slot order), no other legs")
     ap.add_argument("--profile-pair", type=int, default=0,
                     help="profiling mode: the benchmark's autotuning and warm-up, then the "
                          "pair kernel alone N times (the launches the roofline figure "
@@ -164,6 +168,9 @@ def cpu_baseline(n_steps, warmup=10, runs=3, omp_steps=2):
 LADDER_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=4, pair_waves_per_group=1,
                      cell_cells_per_dim=20, sort_interval=20, use_graph=0, energy_every_step=0,
                      side_fork=1)
+SINGLE_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=32, pair_waves_per_group=1,
+                     cell_cells_per_dim=20, sort_interval=20, use_graph=1, energy_every_step=0,
+                     side_fork=4)
 LARGE_TUNING = dict(pair_group=0, pair_newton3=1, pair_split=4, pair_waves_per_group=1,
                     cell_cells_per_dim=33, sort_interval=20, energy_every_step=0, side_fork=1)
 
@@ -206,6 +213,23 @@ def main():
         e5 = big.energies()
         print(json.dumps({"profile_large_steps": a.profile_large, "step": e5[0]["step"],
                           "flags": [x["flags"] for x in e5], "tuning": big.get_tuning()}))
+        return
+    if a.profile_single:
+        s2 = systems.make_config2(frame=0)
+        one = _native.NativeContext(s2, 1)
+        one.set_params(0, systems.params_config3(22)[11])
+        one.set_positions(0, s2.positions)
+        one.set_velocities(0, s2.velocities)
+        one.set_tuning(**SINGLE_TUNING)
+        one.step(max(a.equilibrate, 100))
+        one.energies()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one.step(a.profile_single)
+        e1 = one.energies()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"profile_single_steps": a.profile_single, "steps_per_s": a.profile_single / dt,
+                          "flags": e1[0]["flags"], "tuning": one.get_tuning()}))
         return
     R = a.replicas
     frames = [systems.make_config2(frame=k) for k in range(R)]
